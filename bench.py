@@ -1,0 +1,184 @@
+"""Benchmark of the D2Q9 step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (the largest single-GPU configuration the headline metric is quoted
+on): D2Q9 BGK at 8192 x 4096, tau 0.53, inflow 0.08, 200 seeded circles with
+radius 8..64 (seed 2024), walls on rows 0 and 4095, the reference's outlet.
+A "step" is one lattice update of the whole grid.  The state is resident in HBM
+before the timed region; K steps run between two stream synchronisations.
+
+Printed: ONE JSON line with the FP32 MLUPS as `value`, the roofline of the
+fused step kernel (72 algorithmic bytes per cell update, average kernel
+duration from HIP events recorded on the grid's own stream), the same for
+FP16-shifted storage (36 B per cell update), and the CPU baseline: the oracle's
+faithful f32 restatement of the reference's eight passes, timed on a bounded
+sample of the same workload on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS & %HBM roofline, FP32 vs FP16-store; surrogate speedup at ≤1% u-L2 err"
+HBM_PEAK_GBS = 8000.0
+SEED = 2024
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def read_traffic(name):
+    """HBM bytes per launch of the step kernel from a committed rocprofv3 summary."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), flags=re.M)
+    return float(m.group(1)) if m else None
+
+
+def run_gpu(precision, X, Y, mask, steps, warmup, omega, u_in):
+    from lbm_amd import LBM
+    g = LBM(omega, X, Y, precision=precision)
+    g.mask = mask
+    g.custom_speed(u_in)
+    g.step(warmup)
+    g.sync()
+    t0 = time.perf_counter()
+    dev_ms = g.step_timed(steps)  # waits for the last launch
+    wall = time.perf_counter() - t0
+    cons = g.conservation()
+    g.close()
+    return wall, dev_ms, cons
+
+
+def cpu_baseline(X, Y, mask, omega, u_in, budget_s):
+    import numpy as np
+
+    import oracle
+    threads = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
+    r = oracle.RefCPU(X, Y, omega, threads=threads)
+    r.set_mask(mask.astype(np.uint32))
+    r.reset_equilibrium(1.0, u_in, 0.0)
+    r.step(1)
+    t0 = time.perf_counter()
+    r.step(1)
+    t1 = time.perf_counter() - t0
+    n = max(2, min(400, int(budget_s / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    r.step(n)
+    t = time.perf_counter() - t0
+    del r
+    return {
+        "value": round(X * Y * n / t / 1e6, 2),
+        "unit": "MLUPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle refcpu (f32 restatement of the reference's 8 WGSL passes, OpenMP, one "
+                  f"parallel loop per pass) on the same {X}x{Y} workload, {n} steps after 2 warm-up "
+                  f"steps, {t:.1f} s wall",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--nx", type=int, default=8192)
+    ap.add_argument("--ny", type=int, default=4096)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-f16", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the D2Q9 step here runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not part of this build).", file=sys.stderr)
+        sys.exit(2)
+
+    from lbm_amd import PRECISION_F16S, PRECISION_F32, gen_circles
+
+    X, Y = args.nx, args.ny
+    omega, u_in = 1.0 / 0.53, 0.08
+    cells = X * Y
+    log(f"obstacles: 200 circles r 8..64 seed {SEED} on {X}x{Y}")
+    mask = gen_circles(X, Y, SEED, 200, 8, 64)
+    solid_frac = float(mask[1:-1].mean())
+
+    log(f"f32: warmup {args.warmup}, timed {args.steps}")
+    wall, dev_ms, cons = run_gpu(PRECISION_F32, X, Y, mask, args.steps, args.warmup, omega, u_in)
+    mlups = cells * args.steps / wall / 1e6
+    kern_s = dev_ms / 1e3 / args.steps
+    achieved = 72.0 * cells / kern_s / 1e9
+    log(f"f32: {mlups:.0f} MLUPS, kernel {kern_s * 1e6:.1f} us, {achieved:.0f} GB/s")
+    traffic = read_traffic("traffic_f32_8192x4096.txt") if (X, Y) == (8192, 4096) else None
+
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"D2Q9 BGK FP32 {X}x{Y}, 200 seeded circles r 8-64 (seed {SEED}), tau 0.53, "
+                        f"u_in 0.08, reference outlet",
+            "nx": X, "ny": Y, "tau": 0.53, "u_in": u_in, "solid_fraction": round(solid_frac, 4),
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": 72 * cells,
+            "kernel_avg_us": round(kern_s * 1e6, 2),
+        },
+        "fp64_mass_momentum": [float(c) for c in cons],
+    }
+
+    if not args.no_f16:
+        log("f16s: same workload, FP16-shifted storage")
+        w16, d16, c16 = run_gpu(PRECISION_F16S, X, Y, mask, args.steps, args.warmup, omega, u_in)
+        m16 = cells * args.steps / w16 / 1e6
+        k16 = d16 / 1e3 / args.steps
+        a16 = 36.0 * cells / k16 / 1e9
+        log(f"f16s: {m16:.0f} MLUPS, kernel {k16 * 1e6:.1f} us, {a16:.0f} GB/s")
+        out["f16s"] = {
+            "mlups": round(m16, 1),
+            "speedup_vs_f32": round(m16 / mlups, 3),
+            "roofline": {"bound": "hbm", "achieved": round(a16, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(a16 / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": 36 * cells,
+                         "kernel_avg_us": round(k16 * 1e6, 2),
+                         "traffic": read_traffic("traffic_f16s_8192x4096.txt") if (X, Y) == (8192, 4096)
+                         else None},
+            "fp64_mass_momentum": [float(c) for c in c16],
+        }
+
+    if not args.no_cpu_baseline:
+        log(f"cpu baseline: ~{args.cpu_budget:.0f} s of refcpu")
+        out["cpu_baseline"] = cpu_baseline(X, Y, mask, omega, u_in, args.cpu_budget)
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

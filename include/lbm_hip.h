@@ -1,0 +1,201 @@
+/*
+ * lbm_hip.h — C ABI of the MI355X-native D2Q9 lattice Boltzmann step.
+ *
+ * The library replaces the simulation core of the lbm-wgpu crate (the Rust type
+ * `LBM` in lbm-wgpu/src/lbm.rs and its eight WGSL compute passes) with one fused
+ * HIP pull-stream kernel per step on a gfx950 GPU.  Each entry point below names
+ * the reference method it replaces.  Plain C types only: no HIP or torch types
+ * cross this boundary, and no C++ exception escapes it.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative LBM_E* code; the text of
+ *     the last error on a grid is returned by lbm_last_error(grid) (or
+ *     lbm_last_error(NULL) for a failed lbm_create).
+ *   - A grid owns its device memory and one HIP stream.  Host pointers are
+ *     borrowed for the duration of the call only.  Calls on one grid must be
+ *     serialised by the caller; distinct grids may be used from distinct threads.
+ *   - Cells are indexed x + y * nx (row-major, row y = 0 at the top), like the
+ *     reference (lbm-wgpu/src/lbm.rs:1517-1519).  Host planes are dense nx * ny.
+ *   - Populations are numbered in the reference's plane order
+ *     k = 0 NW, 1 N, 2 NE, 3 W, 4 C (rest), 5 E, 6 SW, 7 S, 8 SE
+ *     (lbm-wgpu/src/lbm.rs:632-640); N moves to row y-1.
+ *   - jx/jy are momenta; jy is north-positive (N - S), as in the reference's
+ *     density bind group (rewritten_shaders/pre_collision/{corner,cardinal}_pre_collision.wgsl).
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------- */
+#define LBM_OK 0
+#define LBM_EINVAL (-1)     /* invalid argument (size, index, enum, null pointer) */
+#define LBM_EHIP (-2)       /* a HIP runtime call failed */
+#define LBM_ENOMEM (-3)     /* device or host allocation failed */
+#define LBM_ENONFINITE (-4) /* a health check found NaN or Inf */
+#define LBM_ENODEV (-5)     /* no usable gfx950 device */
+
+/* ---- enums ----------------------------------------------------------------- */
+/* storage of the distributions; arithmetic is always f32 */
+#define LBM_PRECISION_F32 0  /* 9 x f32 per cell, 72 B per cell update */
+#define LBM_PRECISION_F16S 1 /* f_k - w_k as binary16 (round to nearest even), 36 B */
+
+/* outlet treatment */
+#define LBM_BOUNDARY_REFERENCE 0     /* the reference's flattened-index wrap at x = nx-1 */
+#define LBM_BOUNDARY_ZERO_GRADIENT 1 /* zero-gradient outflow at x = nx-1 */
+
+/* summary fields, in the order of the reference enum SummaryStat (lbm.rs:10-16) */
+#define LBM_SUMMARY_CURL 0
+#define LBM_SUMMARY_UX 1
+#define LBM_SUMMARY_UY 2
+#define LBM_SUMMARY_RHO 3
+#define LBM_SUMMARY_SPEED 4
+
+typedef struct lbm_grid lbm_grid;
+
+typedef struct lbm_config {
+    uint32_t nx, ny;      /* grid size; nx, ny >= 3 */
+    float omega;          /* BGK relaxation rate, 1/tau */
+    float u_init;         /* initial uniform x velocity (LBM::new uses 0.1, lbm.rs:740) */
+    int32_t precision;    /* LBM_PRECISION_* */
+    int32_t boundary;     /* LBM_BOUNDARY_* */
+    int32_t device;       /* HIP device ordinal */
+    uint32_t row_pad;     /* extra elements per row beyond the 64-aligned width (0 = default) */
+} lbm_config;
+
+/* ---- lifecycle --------------------------------------------------------------- */
+
+/* Fill *cfg with defaults: omega 1/(0.5 + 0.3) (lib.rs:39), u_init 0.1, f32,
+ * reference boundary, device 0. */
+int lbm_config_default(lbm_config *cfg, uint32_t nx, uint32_t ny);
+
+/* Replaces LBM::new (lbm.rs:726-1049): allocates the two population sets and the
+ * mask, fills both sets with feq(rho 1, u (u_init, 0)) (lbm.rs:740-745), paints
+ * rows 0 and ny-1 solid (init_barrier, lbm.rs:595-605), zeroes the fields and
+ * the step counter. */
+int lbm_create(const lbm_config *cfg, lbm_grid **out);
+int lbm_destroy(lbm_grid *g);
+const char *lbm_last_error(const lbm_grid *g);
+
+/* ---- parameters and resets --------------------------------------------------- */
+
+/* Replaces LBM::update_omega_buffer (lbm.rs:1358-1360). */
+int lbm_set_omega(lbm_grid *g, float omega);
+/* omega = 1 / (3 nu + 0.5), as the viscosity slider does (lib.rs:181-186). */
+int lbm_set_viscosity(lbm_grid *g, float nu);
+int lbm_get_omega(const lbm_grid *g, float *omega);
+
+/* Replaces LBM::reset_to_equilibrium / LBM::custom_speed (lbm.rs:1076-1102):
+ * both sets := feq(rho, (ux, uy)) computed like set_equil (lbm.rs:611-643), the
+ * step counter := 0, and the fields := the pre-collision moments of that state
+ * (which, as in the reference, leave the rest population out of rho). */
+int lbm_reset_equilibrium(lbm_grid *g, float rho, float ux, float uy);
+
+/* Replaces LBM::single_cell (lbm.rs:1482-1515): both sets := feq(1, 0) with the
+ * value 4.0 in plane k at the reference's fixed cell for k; step counter := 0;
+ * the fields are left as they are.  k outside 0..8 leaves a plain rest state. */
+int lbm_single_cell(lbm_grid *g, int k);
+
+/* Set population k of the current state at (x, y) to v (no reference
+ * counterpart; a general form of single_cell's pulse). */
+int lbm_set_pulse(lbm_grid *g, int k, uint32_t x, uint32_t y, float v);
+
+/* ---- obstacles ---------------------------------------------------------------- */
+
+/* Replaces LBM::reset_barrier (lbm.rs:1362-1365): only rows 0 and ny-1 solid. */
+int lbm_reset_barrier(lbm_grid *g);
+
+/* Replaces LBM::draw_shape / draw_barrier_updates (lbm.rs:1337-1356) with the
+ * same wire format as get_points_vector (barrier_shapes/merge_shapes.rs:12-22):
+ * n_u32 words [idx0, val0, idx1, val1, ...], idx = x + y * nx, and a cell is solid
+ * iff its value is 1 (stream/e_w_stream.wgsl:31).  Pairs apply in list order, so the
+ * last pair for a cell wins.  An odd length or an index >= nx*ny is LBM_EINVAL
+ * and changes nothing; an empty list is a no-op (the reference underflows on it,
+ * lbm.rs:1343).  Populations are not touched. */
+int lbm_paint(lbm_grid *g, const uint32_t *pairs, size_t n_u32);
+
+/* Whole-mask upload / download, one byte per cell, 1 = solid. */
+int lbm_set_mask(lbm_grid *g, const uint8_t *solid);
+int lbm_get_mask(lbm_grid *g, uint8_t *solid);
+
+/* ---- stepping ------------------------------------------------------------------ */
+
+/* Replaces LBM::iterate's loop over compute_step (lbm.rs:1065-1068, 1112-1134):
+ * enqueues n steps on the grid's stream and returns without waiting.  The
+ * fields of the last step (the reference's density bind group) are written by
+ * the last launch. */
+int lbm_step(lbm_grid *g, uint32_t n);
+/* Blocks until the grid's stream is idle. */
+int lbm_sync(lbm_grid *g);
+/* lbm_step(g, n) bracketed by two hipEvents recorded on the grid's stream;
+ * waits for the second and returns the elapsed device time in *ms.  (No
+ * reference counterpart: the reference has no timing, lbm.rs:1166-1172.) */
+int lbm_step_timed(lbm_grid *g, uint32_t n, float *ms);
+/* Replaces LBM::get_compute_num (lbm.rs:1170-1172). */
+int lbm_step_count(const lbm_grid *g, uint64_t *n);
+/* The grid's hipStream_t, as an opaque pointer. */
+int lbm_get_stream(const lbm_grid *g, void **stream);
+
+/* ---- fields ---------------------------------------------------------------------- */
+
+/* Momentum and density planes.  lagged != 0: the renderer's view, i.e. the
+ * pre-collision moments computed in the last executed step (one step behind the
+ * state; lbm.rs:1065-1074).  lagged == 0: the moments of the current state. */
+int lbm_read_moments(lbm_grid *g, float *jx, float *jy, float *rho, int lagged);
+
+/* rho, velocity u = j / rho and the reference's curl
+ * 10 (jy[x+1] - jy[x-1] - jx[y-1] + jx[y+1]) / rho (summary_stats/curl.wgsl:49),
+ * which is 0 where the reference skips it (x = 0, y = ny-1).  Any pointer may be
+ * NULL. */
+int lbm_read_fields(lbm_grid *g, float *rho, float *ux, float *uy, float *curl, int lagged);
+
+/* Replaces LBM::calculate_summary (lbm.rs:1051-1059): one renderer field into a
+ * persistent output plane that, like the reference's, keeps values the pass
+ * skips; kind is LBM_SUMMARY_*. */
+int lbm_summary(lbm_grid *g, int kind, float *out);
+
+/* FP64 sums of rho, jx, jy over the fluid cells (mask != 1) of the current
+ * state.  Returns LBM_ENONFINITE if any sum is NaN or Inf. */
+int lbm_conservation(lbm_grid *g, double out[3]);
+
+/* ---- state ------------------------------------------------------------------------- */
+
+/* The current state S_n as 9 dense f32 planes (9 * nx * ny floats): the
+ * reference's planes k != 4 of set n % 2 plus the rest plane (lbm.rs:775). */
+int lbm_read_f(lbm_grid *g, float *f9);
+/* Write S_n := f9 into both sets (as the reference's resets do); the step count
+ * and the fields are left as they are. */
+int lbm_write_f(lbm_grid *g, const float *f9);
+
+/* Checkpoint: an opaque, self-describing byte image of the whole grid state
+ * (both sets, mask, fields, omega, step count).  Loading it into a grid of the
+ * same size, precision and boundary mode continues bit for bit. */
+int lbm_state_size(const lbm_grid *g, size_t *bytes);
+int lbm_save_state(lbm_grid *g, void *buf, size_t bytes);
+int lbm_load_state(lbm_grid *g, const void *buf, size_t bytes);
+
+/* ---- seeded obstacle generator (host only, no GPU) --------------------------------- */
+
+/* SplitMix64 (Steele, Lea and Flood 2014).  Returns the next 64-bit output and
+ * advances *state. */
+uint64_t lbm_splitmix64(uint64_t *state);
+
+/* Paint n_circles filled circles into solid[] (nx * ny bytes; existing bytes are
+ * kept, so callers pass a zeroed or wall-painted array).  For each circle, in
+ * this order: r = r_min + u(r_max - r_min + 1), cx = r + 1 + u(nx - r - 1),
+ * cy = u(ny), with u(m) = (splitmix64() >> 32) * m >> 32.  Cells with
+ * (x - cx)^2 + (y - cy)^2 <= r^2 become 1; circles are clipped at the grid edge
+ * and keep column 0 free (cx - r >= 1). */
+int lbm_gen_circles(uint8_t *solid, uint32_t nx, uint32_t ny, uint64_t seed,
+                    uint32_t n_circles, uint32_t r_min, uint32_t r_max);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_HIP_H */

@@ -1,0 +1,1067 @@
+// Single-domain D2Q9 lattice Boltzmann engine for gfx950, behind the C ABI in
+// include/lbm_hip.h.
+//
+// State representation
+// --------------------
+// The reference (lbm-wgpu src/lbm.rs:1112-1134) keeps two parity sets of nine
+// planes.  Step n+1 collides set p = n % 2 in place (C_{n+1}) and then streams
+// it into set 1-p; cells the stream does not write (solid cells, column 0 and
+// row ny-1, stream/e_w_stream.wgsl:31-45) keep what set 1-p held.  The state
+// after n steps is therefore the pair (S_n = set n%2, set (n+1)%2).
+//
+// This engine keeps the same two sets but runs each step as ONE pass that pulls
+// and collides in registers ("P-form"):
+//   S-form   buf[cur] = S_n (pre-collision), buf[1-cur] = the other set.
+//   P-form   buf[cur] = C_n = collide(S_{n-1}), buf[1-cur] = the other set; the
+//            stream of step n is pending.
+// Kernels
+//   collide (S -> P)   buf[cur] := collide(buf[cur]) in place.
+//   fused   (P -> P)   S_n := pull(buf[cur]) on streamed cells, buf[1-cur] own
+//                      values elsewhere; buf[1-cur] := collide(S_n); swap.
+//   flush   (P -> S)   buf[1-cur] := S_n (streamed cells only); swap.
+// Since the pull of step n uses the mask at the time the kernel runs, any mask
+// change first flushes a pending stream, so results are those of the reference
+// step order exactly.  The rest population is stored in both sets (the
+// reference keeps one shared plane, lbm.rs:775); it is never streamed, so the
+// values are the same.
+//
+// Memory layout in HBM: structure of arrays, nine planes per set, each plane
+// pitch * ny elements (pitch = nx rounded up to 64 plus a pad), planes
+// consecutive in one allocation per set.  Mask and the per-cell bounce-back byte
+// are u8 planes with the same pitch.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/lbm_hip.h"
+#include "d2q9.h"
+
+using namespace d2q9;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// device side
+// ---------------------------------------------------------------------------
+
+struct Geom {
+    int X, Y;
+    int pitch;          // elements per row
+    long long pstride;  // elements per plane
+    float omega;
+};
+
+// per-cell bounce-back byte: bit(k) set if population k's upstream is solid or
+// outside the grid; 0xFF marks a cell the fast path does not pull (not streamed,
+// or streamed with all eight upstream cells solid)
+constexpr uint8_t kSlowCell = 0xFF;
+
+// upstream cell of population k at (x, y); mode 0 wraps the flattened index at
+// the outlet column, mode 1 clamps it (zero-gradient outflow)
+template <int MODE>
+__host__ __device__ __forceinline__ void upstream(int k, int x, int y, int X, int &ux, int &uy) {
+    ux = x - cx(k);
+    uy = y - cy(k);
+    if (ux >= X) {
+        if (MODE == LBM_BOUNDARY_REFERENCE) {
+            ux = 0;
+            uy += 1;
+        } else {
+            ux = X - 1;
+        }
+    }
+}
+
+template <int MODE>
+__global__ void k_build_flags(const uint8_t *__restrict__ mask, uint8_t *__restrict__ flags, Geom g) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= g.X) return;
+    const size_t own = (size_t)y * g.pitch + x;
+    const bool streamed = mask[own] != 1 && x != 0 && y < g.Y - 1;
+    uint8_t b = kSlowCell;
+    if (streamed) {
+        b = 0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            if (k == C) continue;
+            int ux, uy;
+            upstream<MODE>(k, x, y, g.X, ux, uy);
+            const bool solid = uy < 0 || uy >= g.Y || ux < 0 || mask[(size_t)uy * g.pitch + ux] == 1;
+            if (solid) b |= (uint8_t)(1u << bit(k));
+        }
+    }
+    flags[own] = b;
+}
+
+enum : int {
+    OP_COLLIDE = 1,   // collide the gathered state and store all nine planes to dst
+    OP_STORE_S = 2,   // store the gathered (streamed) state to dst
+    OP_MOMENTS = 4,   // write jx, jy, rho of the gathered state
+    OP_REDUCE = 8,    // FP64 partial sums of rho, jx, jy over fluid cells
+    OP_NOREST = 16,   // moments without the rest population
+};
+
+constexpr int kBlock = 256;
+
+template <class St, int MODE, bool PFORM, int OP>
+__global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, typename St::T *dst,
+                                                 const uint8_t *__restrict__ flags,
+                                                 const uint8_t *__restrict__ mask, Geom g,
+                                                 float *__restrict__ mom, double *__restrict__ partials) {
+    using T = typename St::T;
+    const int x = blockIdx.x * kBlock + threadIdx.x;
+    const int y = blockIdx.y;
+    const bool valid = x < g.X;
+    if (!(OP & OP_REDUCE) && !valid) return;
+    const size_t own = (size_t)y * g.pitch + (valid ? x : 0);
+    const long long ps = g.pstride;
+
+    float f[9];
+    bool streamed = false;
+    if (valid) {
+        if (PFORM) {
+            const uint8_t fl = flags[own];
+            if (fl != kSlowCell) {
+                streamed = true;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    if (k == C) continue;
+                    if (fl & (1u << bit(k))) {
+                        f[k] = St::decode(src[opp(k) * ps + own], opp(k));
+                    } else {
+                        int ux, uy;
+                        upstream<MODE>(k, x, y, g.X, ux, uy);
+                        f[k] = St::decode(src[k * ps + (size_t)uy * g.pitch + ux], k);
+                    }
+                }
+            } else {
+                streamed = mask[own] != 1 && x != 0 && y < g.Y - 1;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    if (k == C) continue;
+                    // enclosed streamed cell: every population bounces back;
+                    // otherwise the cell is not streamed and keeps the other set
+                    f[k] = streamed ? St::decode(src[opp(k) * ps + own], opp(k))
+                                    : St::decode(dst[k * ps + own], k);
+                }
+            }
+            f[C] = St::decode(src[C * ps + own], C);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) f[k] = St::decode(src[k * ps + own], k);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f[k] = 0.0f;
+    }
+
+    Moments m;
+    if (OP & OP_STORE_S) {
+        if (streamed) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                if (k != C) dst[k * ps + own] = St::encode(f[k], k);
+        }
+        dst[C * ps + own] = src[C * ps + own];
+    }
+    if (OP & OP_COLLIDE) {
+        m = collide_ref(f, g.omega);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) dst[k * ps + own] = St::encode(f[k], k);
+    } else {
+        m = moments_ref(f, !(OP & OP_NOREST));
+    }
+    if (OP & OP_MOMENTS) {
+        if (valid) {
+            mom[own] = m.jx;
+            mom[ps + own] = m.jy;
+            mom[2 * ps + own] = m.rho;
+        }
+    }
+    if (OP & OP_REDUCE) {
+        const bool fluid = valid && mask[own] != 1;
+        double s0 = fluid ? (double)m.rho : 0.0;
+        double s1 = fluid ? (double)m.jx : 0.0;
+        double s2 = fluid ? (double)m.jy : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            s0 += __shfl_xor(s0, off, 64);
+            s1 += __shfl_xor(s1, off, 64);
+            s2 += __shfl_xor(s2, off, 64);
+        }
+        __shared__ double red[kBlock / 64][3];
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        if (lane == 0) {
+            red[wv][0] = s0;
+            red[wv][1] = s1;
+            red[wv][2] = s2;
+        }
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            double t = 0.0;
+            for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
+            partials[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = t;
+        }
+    }
+}
+
+// one block sums the per-block partials in a fixed order (deterministic)
+__global__ __launch_bounds__(1024) void k_reduce_final(const double *__restrict__ partials, size_t n,
+                                                       double *__restrict__ out) {
+    __shared__ double red[1024 / 64][3];
+    double s[3] = {0.0, 0.0, 0.0};
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) {
+        s[0] += partials[i * 3 + 0];
+        s[1] += partials[i * 3 + 1];
+        s[2] += partials[i * 3 + 2];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        for (int off = 32; off > 0; off >>= 1) s[c] += __shfl_xor(s[c], off, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int c = 0; c < 3; ++c) red[wv][c] = s[c];
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w) t += red[w][threadIdx.x];
+        out[threadIdx.x] = t;
+    }
+}
+
+template <class St>
+__global__ void k_fill(typename St::T *buf, long long pstride, float4 v0123, float4 v4567, float v8) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pstride) return;
+    const float v[9] = {v0123.x, v0123.y, v0123.z, v0123.w, v4567.x, v4567.y, v4567.z, v4567.w, v8};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) buf[k * pstride + i] = St::encode(v[k], k);
+}
+
+template <class St>
+__global__ void k_encode_plane(const float *__restrict__ dense, typename St::T *__restrict__ plane, Geom g, int k) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= g.X) return;
+    plane[(size_t)y * g.pitch + x] = St::encode(dense[(size_t)y * g.X + x], k);
+}
+
+template <class St>
+__global__ void k_decode_plane(const typename St::T *__restrict__ plane, float *__restrict__ dense, Geom g, int k) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= g.X) return;
+    dense[(size_t)y * g.X + x] = St::decode(plane[(size_t)y * g.pitch + x], k);
+}
+
+template <class St>
+__global__ void k_set_one(typename St::T *p, float v, int k) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = St::encode(v, k);
+}
+
+__global__ void k_paint(const uint32_t *__restrict__ idx, const uint8_t *__restrict__ val, size_t n,
+                        uint8_t *__restrict__ mask, int X, int pitch) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = idx[i];
+    mask[(size_t)(c / X) * pitch + (c % X)] = val[i];
+}
+
+__global__ void k_walls(uint8_t *__restrict__ mask, Geom g) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= g.X) return;
+    mask[x] = 1;
+    mask[(size_t)(g.Y - 1) * g.pitch + x] = 1;
+}
+
+// summary passes (summary_stats/{curl,ux,uy,rho,speed}.wgsl); out keeps skipped cells
+__global__ void k_summary(const float *__restrict__ mom, float *__restrict__ out, Geom g, int kind) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= g.X) return;
+    const long long ps = g.pstride;
+    const float *jx = mom, *jy = mom + ps, *rho = mom + 2 * ps;
+    const size_t i = (size_t)y * g.pitch + x;
+    float v;
+    switch (kind) {
+    case LBM_SUMMARY_CURL: {
+        if (x == 0 || y >= g.Y - 1) return;
+        // flattened neighbours: x+1 at the last column is column 0 of the next row
+        const int xe = x + 1 < g.X ? x + 1 : 0, ye = x + 1 < g.X ? y : y + 1;
+        const float uye = jy[(size_t)ye * g.pitch + xe];
+        const float uyw = jy[i - 1];
+        const float uxn = y > 0 ? jx[i - g.pitch] : 0.0f;
+        const float uxs = jx[i + g.pitch];
+        v = 10.0f * (uye - uyw - uxn + uxs) / rho[i];
+        break;
+    }
+    case LBM_SUMMARY_UX: v = jx[i]; break;
+    case LBM_SUMMARY_UY: v = jy[i]; break;
+    case LBM_SUMMARY_RHO: v = 4.0f * fminf(fmaxf(0.15f * rho[i], 0.0f), 1.0f) - 0.5f; break;
+    default: v = fminf(fmaxf(5.0f * sqrtf(jx[i] * jx[i] + jy[i] * jy[i]), 0.0f), 1.0f) - 0.5f; break;
+    }
+    out[i] = v;
+}
+
+// rho, u = j / rho, curl into four dense planes
+__global__ void k_fields(const float *__restrict__ mom, float *__restrict__ outp, Geom g) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= g.X) return;
+    const long long ps = g.pstride;
+    const float *jx = mom, *jy = mom + ps, *rho = mom + 2 * ps;
+    const size_t i = (size_t)y * g.pitch + x;
+    const size_t d = (size_t)y * g.X + x, dn = (size_t)g.X * g.Y;
+    const float r = rho[i];
+    outp[d] = r;
+    outp[dn + d] = jx[i] / r;
+    outp[2 * dn + d] = jy[i] / r;
+    float curl = 0.0f;
+    if (x != 0 && y < g.Y - 1) {
+        const int xe = x + 1 < g.X ? x + 1 : 0, ye = x + 1 < g.X ? y : y + 1;
+        const float uye = jy[(size_t)ye * g.pitch + xe];
+        const float uyw = jy[i - 1];
+        const float uxn = y > 0 ? jx[i - g.pitch] : 0.0f;
+        const float uxs = jx[i + g.pitch];
+        curl = 10.0f * (uye - uyw - uxn + uxs) / r;
+    }
+    outp[3 * dn + d] = curl;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+thread_local std::string g_create_error;
+
+struct HipError {
+    hipError_t e;
+    const char *what;
+};
+
+#define HIPCHK(expr)                                  \
+    do {                                              \
+        hipError_t e_ = (expr);                       \
+        if (e_ != hipSuccess) throw HipError{e_, #expr}; \
+    } while (0)
+
+struct ArgError {
+    std::string msg;
+};
+
+template <class T>
+T *dalloc(size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) throw HipError{e, "hipMalloc"};
+    return (T *)p;
+}
+
+}  // namespace
+
+struct lbm_grid {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    uint32_t X = 0, Y = 0;
+    int precision = 0, boundary = 0;
+    Geom geom{};
+    size_t esize = 4;
+    float omega = 1.25f;
+    void *f[2] = {nullptr, nullptr};
+    int cur = 0;
+    bool pform = false;
+    uint8_t *mask = nullptr, *flags = nullptr;
+    float *lag = nullptr;   // jx, jy, rho planes: the renderer's density bind group
+    float *now = nullptr;   // moments of the current state (scratch)
+    float *out = nullptr;   // summary output plane (persistent)
+    float *dense = nullptr; // dense f32 staging (4 planes)
+    double *partials = nullptr;
+    double *sums = nullptr;
+    size_t npartials = 0;
+    uint64_t steps = 0;
+    std::string err;
+
+    size_t plane_bytes() const { return (size_t)geom.pstride * esize; }
+    size_t set_bytes() const { return 9 * plane_bytes(); }
+    dim3 cell_grid() const { return dim3((X + kBlock - 1) / kBlock, Y); }
+
+    void release() {
+        (void)hipSetDevice(dev);
+        for (void *p : {f[0], f[1], (void *)mask, (void *)flags, (void *)lag, (void *)now, (void *)out,
+                        (void *)dense, (void *)partials, (void *)sums})
+            if (p) (void)hipFree(p);
+        f[0] = f[1] = nullptr;
+        mask = flags = nullptr;
+        lag = now = out = dense = nullptr;
+        partials = sums = nullptr;
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+
+    // ---- kernels launchers -------------------------------------------------
+    template <class St, bool PFORM, int OP>
+    void launch_cell(const void *src, void *dst, float *mom, double *part) {
+        using T = typename St::T;
+        if (boundary == LBM_BOUNDARY_REFERENCE)
+            hipLaunchKernelGGL((k_cell<St, LBM_BOUNDARY_REFERENCE, PFORM, OP>), cell_grid(), dim3(kBlock), 0,
+                               stream, (const T *)src, (T *)dst, flags, mask, geom, mom, part);
+        else
+            hipLaunchKernelGGL((k_cell<St, LBM_BOUNDARY_ZERO_GRADIENT, PFORM, OP>), cell_grid(), dim3(kBlock), 0,
+                               stream, (const T *)src, (T *)dst, flags, mask, geom, mom, part);
+        HIPCHK(hipGetLastError());
+    }
+
+    template <bool PFORM, int OP>
+    void cell(const void *src, void *dst, float *mom, double *part) {
+        if (precision == LBM_PRECISION_F32)
+            launch_cell<StoreF32, PFORM, OP>(src, dst, mom, part);
+        else
+            launch_cell<StoreF16S, PFORM, OP>(src, dst, mom, part);
+    }
+
+    void build_flags() {
+        dim3 grid((X + 255) / 256, Y);
+        if (boundary == LBM_BOUNDARY_REFERENCE)
+            hipLaunchKernelGGL(k_build_flags<LBM_BOUNDARY_REFERENCE>, grid, dim3(256), 0, stream, mask, flags, geom);
+        else
+            hipLaunchKernelGGL(k_build_flags<LBM_BOUNDARY_ZERO_GRADIENT>, grid, dim3(256), 0, stream, mask, flags,
+                               geom);
+        HIPCHK(hipGetLastError());
+    }
+
+    void fill_both(const float v[9]) {
+        const float4 a = make_float4(v[0], v[1], v[2], v[3]);
+        const float4 b = make_float4(v[4], v[5], v[6], v[7]);
+        const dim3 grid((unsigned)((geom.pstride + 255) / 256));
+        for (int s = 0; s < 2; ++s) {
+            if (precision == LBM_PRECISION_F32)
+                hipLaunchKernelGGL(k_fill<StoreF32>, grid, dim3(256), 0, stream, (float *)f[s], geom.pstride, a, b,
+                                   v[8]);
+            else
+                hipLaunchKernelGGL(k_fill<StoreF16S>, grid, dim3(256), 0, stream, (uint16_t *)f[s], geom.pstride,
+                                   a, b, v[8]);
+            HIPCHK(hipGetLastError());
+        }
+    }
+
+    void ensure_now() {
+        if (!now) now = dalloc<float>(3 * (size_t)geom.pstride);
+    }
+    void ensure_dense() {
+        if (!dense) dense = dalloc<float>(4 * (size_t)X * Y);
+    }
+    void ensure_partials() {
+        if (!partials) {
+            const dim3 gr = cell_grid();
+            npartials = (size_t)gr.x * gr.y;
+            partials = dalloc<double>(3 * npartials);
+            sums = dalloc<double>(3);
+        }
+    }
+
+    // P-form -> S-form: complete the pending stream with the current mask
+    void flush() {
+        if (!pform) return;
+        cell<true, OP_STORE_S>(f[cur], f[1 - cur], nullptr, nullptr);
+        cur = 1 - cur;
+        pform = false;
+    }
+
+    void step(uint32_t n) {
+        for (uint32_t i = 0; i < n; ++i) {
+            const bool last = i + 1 == n;
+            if (!pform) {
+                if (last)
+                    cell<false, OP_COLLIDE | OP_MOMENTS>(f[cur], f[cur], lag, nullptr);
+                else
+                    cell<false, OP_COLLIDE>(f[cur], f[cur], nullptr, nullptr);
+                pform = true;
+            } else {
+                if (last)
+                    cell<true, OP_COLLIDE | OP_MOMENTS>(f[cur], f[1 - cur], lag, nullptr);
+                else
+                    cell<true, OP_COLLIDE>(f[cur], f[1 - cur], nullptr, nullptr);
+                cur = 1 - cur;
+            }
+        }
+        steps += n;
+    }
+
+    // moments of the current state into `now` (optionally with the FP64 reduction)
+    void moments_now(bool reduce) {
+        ensure_now();
+        if (reduce) ensure_partials();
+        if (pform) {
+            if (reduce) cell<true, OP_MOMENTS | OP_REDUCE>(f[cur], f[1 - cur], now, partials);
+            else cell<true, OP_MOMENTS>(f[cur], f[1 - cur], now, nullptr);
+        } else {
+            if (reduce) cell<false, OP_MOMENTS | OP_REDUCE>(f[cur], nullptr, now, partials);
+            else cell<false, OP_MOMENTS>(f[cur], nullptr, now, nullptr);
+        }
+    }
+
+    void copy_plane_out(const void *dev_plane, float *host, int k) {
+        // decode one pitched population plane into a dense f32 host plane
+        ensure_dense();
+        dim3 grid((X + 255) / 256, Y);
+        if (precision == LBM_PRECISION_F32)
+            hipLaunchKernelGGL(k_decode_plane<StoreF32>, grid, dim3(256), 0, stream, (const float *)dev_plane, dense,
+                               geom, k);
+        else
+            hipLaunchKernelGGL(k_decode_plane<StoreF16S>, grid, dim3(256), 0, stream, (const uint16_t *)dev_plane,
+                               dense, geom, k);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(host, dense, sizeof(float) * X * Y, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    void copy_plane_in(const float *host, void *dev_plane, int k) {
+        ensure_dense();
+        HIPCHK(hipMemcpyAsync(dense, host, sizeof(float) * X * Y, hipMemcpyHostToDevice, stream));
+        dim3 grid((X + 255) / 256, Y);
+        if (precision == LBM_PRECISION_F32)
+            hipLaunchKernelGGL(k_encode_plane<StoreF32>, grid, dim3(256), 0, stream, dense, (float *)dev_plane, geom,
+                               k);
+        else
+            hipLaunchKernelGGL(k_encode_plane<StoreF16S>, grid, dim3(256), 0, stream, dense, (uint16_t *)dev_plane,
+                               geom, k);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    void *plane(int s, int k) { return (char *)f[s] + (size_t)k * plane_bytes(); }
+
+    void copy_f32_plane_out(const float *dev, float *host) {
+        HIPCHK(hipMemcpy2DAsync(host, sizeof(float) * X, dev, sizeof(float) * geom.pitch, sizeof(float) * X, Y,
+                                hipMemcpyDeviceToHost, stream));
+    }
+};
+
+namespace {
+
+int fail(lbm_grid *g, int code, const std::string &msg) {
+    if (g) g->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+template <class F>
+int guarded(lbm_grid *g, F &&fn) {
+    if (!g) return fail(nullptr, LBM_EINVAL, "null grid");
+    try {
+        HIPCHK(hipSetDevice(g->dev));
+        fn();
+        g->err.clear();
+        return LBM_OK;
+    } catch (const HipError &e) {
+        return fail(g, e.e == hipErrorOutOfMemory ? LBM_ENOMEM : LBM_EHIP,
+                    std::string(e.what) + ": " + hipGetErrorString(e.e));
+    } catch (const ArgError &e) {
+        return fail(g, LBM_EINVAL, e.msg);
+    } catch (const std::bad_alloc &) {
+        return fail(g, LBM_ENOMEM, "host allocation failed");
+    } catch (...) {
+        return fail(g, LBM_EHIP, "unexpected exception");
+    }
+}
+
+void check(bool ok, const char *msg) {
+    if (!ok) throw ArgError{msg};
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int lbm_config_default(lbm_config *cfg, uint32_t nx, uint32_t ny) {
+    if (!cfg) return LBM_EINVAL;
+    cfg->nx = nx;
+    cfg->ny = ny;
+    cfg->omega = 1.0f / (0.5f + 0.3f);
+    cfg->u_init = 0.1f;
+    cfg->precision = LBM_PRECISION_F32;
+    cfg->boundary = LBM_BOUNDARY_REFERENCE;
+    cfg->device = 0;
+    cfg->row_pad = 0;
+    return LBM_OK;
+}
+
+const char *lbm_last_error(const lbm_grid *g) { return g ? g->err.c_str() : g_create_error.c_str(); }
+
+int lbm_create(const lbm_config *cfg, lbm_grid **out) {
+    if (!cfg || !out) return fail(nullptr, LBM_EINVAL, "null argument");
+    *out = nullptr;
+    if (cfg->nx < 3 || cfg->ny < 3) return fail(nullptr, LBM_EINVAL, "nx and ny must be >= 3");
+    if (cfg->ny > 65535) return fail(nullptr, LBM_EINVAL, "ny must be <= 65535 (one launch row per grid row)");
+    if ((uint64_t)cfg->nx * cfg->ny > 0xFFFFFFFFull)
+        return fail(nullptr, LBM_EINVAL, "nx * ny must fit in 32 bits (the paint wire format is u32)");
+    if (cfg->precision != LBM_PRECISION_F32 && cfg->precision != LBM_PRECISION_F16S)
+        return fail(nullptr, LBM_EINVAL, "unknown precision");
+    if (cfg->boundary != LBM_BOUNDARY_REFERENCE && cfg->boundary != LBM_BOUNDARY_ZERO_GRADIENT)
+        return fail(nullptr, LBM_EINVAL, "unknown boundary mode");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, LBM_ENODEV, "no HIP device");
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(nullptr, LBM_EINVAL, "bad device ordinal");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess)
+        return fail(nullptr, LBM_ENODEV, "hipGetDeviceProperties failed");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(nullptr, LBM_ENODEV, std::string("this library is built for gfx950, device is ") + prop.gcnArchName);
+
+    lbm_grid *g = new (std::nothrow) lbm_grid();
+    if (!g) return fail(nullptr, LBM_ENOMEM, "host allocation failed");
+    g->dev = cfg->device;
+    g->X = cfg->nx;
+    g->Y = cfg->ny;
+    g->precision = cfg->precision;
+    g->boundary = cfg->boundary;
+    g->esize = cfg->precision == LBM_PRECISION_F32 ? 4 : 2;
+    g->omega = cfg->omega;
+    const int pitch = (int)((cfg->nx + 63u) / 64u * 64u + cfg->row_pad);
+    long long pstride = (long long)pitch * cfg->ny;
+    pstride = (pstride + 255) / 256 * 256;
+    g->geom = Geom{(int)cfg->nx, (int)cfg->ny, pitch, pstride, cfg->omega};
+    int rc = guarded(g, [&] {
+        HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+        g->f[0] = dalloc<char>(g->set_bytes());
+        g->f[1] = dalloc<char>(g->set_bytes());
+        g->mask = dalloc<uint8_t>(pstride);
+        g->flags = dalloc<uint8_t>(pstride);
+        g->lag = dalloc<float>(3 * (size_t)pstride);
+        g->out = dalloc<float>(pstride);
+        HIPCHK(hipMemsetAsync(g->lag, 0, sizeof(float) * 3 * pstride, g->stream));
+        HIPCHK(hipMemsetAsync(g->out, 0, sizeof(float) * pstride, g->stream));
+        float v[9];
+        host_feq(cfg->u_init, 0.0f, 1.0f, v);
+        g->fill_both(v);
+        HIPCHK(hipMemsetAsync(g->mask, 0, pstride, g->stream));
+        hipLaunchKernelGGL(k_walls, dim3((g->X + 255) / 256), dim3(256), 0, g->stream, g->mask, g->geom);
+        HIPCHK(hipGetLastError());
+        g->build_flags();
+        HIPCHK(hipStreamSynchronize(g->stream));
+    });
+    if (rc != LBM_OK) {
+        g_create_error = g->err;
+        g->release();
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_grid *g) {
+    if (!g) return LBM_EINVAL;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    g->release();
+    delete g;
+    return LBM_OK;
+}
+
+int lbm_set_omega(lbm_grid *g, float omega) {
+    return guarded(g, [&] {
+        check(std::isfinite(omega), "omega must be finite");
+        g->omega = omega;
+        g->geom.omega = omega;
+    });
+}
+
+int lbm_set_viscosity(lbm_grid *g, float nu) {
+    return lbm_set_omega(g, 1.0f / (3.0f * nu + 0.5f));
+}
+
+int lbm_get_omega(const lbm_grid *g, float *omega) {
+    if (!g || !omega) return LBM_EINVAL;
+    *omega = g->omega;
+    return LBM_OK;
+}
+
+int lbm_reset_equilibrium(lbm_grid *g, float rho, float ux, float uy) {
+    return guarded(g, [&] {
+        float v[9];
+        host_feq(ux, uy, rho, v);
+        g->fill_both(v);
+        g->cur = 0;
+        g->pform = false;
+        g->steps = 0;
+        // the reference runs the two pre-collision passes after a reset
+        g->cell<false, OP_MOMENTS | OP_NOREST>(g->f[0], nullptr, g->lag, nullptr);
+    });
+}
+
+int lbm_single_cell(lbm_grid *g, int k) {
+    return guarded(g, [&] {
+        float v[9];
+        host_feq(0.0f, 0.0f, 1.0f, v);
+        g->fill_both(v);
+        g->cur = 0;
+        g->pform = false;
+        g->steps = 0;
+        const uint32_t X = g->X, Y = g->Y;
+        uint32_t x = 0, y = 0;
+        switch (k) {  // set_single_cell, lbm.rs:1486-1496
+        case 0: x = X - 2; y = Y - 2; break;
+        case 1: x = 3 * X / 4; y = Y - 2; break;
+        case 2: x = X / 3; y = Y - 2; break;
+        case 3: x = X - 2; y = Y / 2; break;
+        case 4: x = 3 * X / 4; y = Y / 2; break;
+        case 5: x = X / 2; y = Y / 2; break;
+        case 6: x = X - 2; y = 1; break;
+        case 7: x = 3 * X / 4; y = 1; break;
+        case 8: x = X / 2; y = 1; break;
+        default: return;
+        }
+        const size_t off = (size_t)y * g->geom.pitch + x;
+        for (int s = 0; s < 2; ++s) {
+            char *p = (char *)g->plane(s, k) + off * g->esize;
+            if (g->precision == LBM_PRECISION_F32)
+                hipLaunchKernelGGL(k_set_one<StoreF32>, dim3(1), dim3(64), 0, g->stream, (float *)p, 4.0f, k);
+            else
+                hipLaunchKernelGGL(k_set_one<StoreF16S>, dim3(1), dim3(64), 0, g->stream, (uint16_t *)p, 4.0f, k);
+            HIPCHK(hipGetLastError());
+        }
+    });
+}
+
+int lbm_set_pulse(lbm_grid *g, int k, uint32_t x, uint32_t y, float v) {
+    return guarded(g, [&] {
+        check(k >= 0 && k < 9, "k must be in 0..8");
+        check(x < g->X && y < g->Y, "cell outside the grid");
+        g->flush();
+        const size_t off = (size_t)y * g->geom.pitch + x;
+        char *p = (char *)g->plane(g->cur, k) + off * g->esize;
+        if (g->precision == LBM_PRECISION_F32)
+            hipLaunchKernelGGL(k_set_one<StoreF32>, dim3(1), dim3(64), 0, g->stream, (float *)p, v, k);
+        else
+            hipLaunchKernelGGL(k_set_one<StoreF16S>, dim3(1), dim3(64), 0, g->stream, (uint16_t *)p, v, k);
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int lbm_reset_barrier(lbm_grid *g) {
+    return guarded(g, [&] {
+        g->flush();
+        HIPCHK(hipMemsetAsync(g->mask, 0, g->geom.pstride, g->stream));
+        hipLaunchKernelGGL(k_walls, dim3((g->X + 255) / 256), dim3(256), 0, g->stream, g->mask, g->geom);
+        HIPCHK(hipGetLastError());
+        g->build_flags();
+    });
+}
+
+int lbm_paint(lbm_grid *g, const uint32_t *pairs, size_t n_u32) {
+    return guarded(g, [&] {
+        if (n_u32 == 0) return;
+        check(pairs != nullptr, "null pairs");
+        check(n_u32 % 2 == 0, "pairs must hold an even number of u32 words");
+        const uint64_t total = (uint64_t)g->X * g->Y;
+        for (size_t j = 0; j < n_u32; j += 2) check(pairs[j] < total, "cell index out of range");
+        // last writer wins: keep the final value per index
+        std::unordered_map<uint32_t, uint8_t> last;
+        last.reserve(n_u32 / 2);
+        for (size_t j = 0; j < n_u32; j += 2) last[pairs[j]] = pairs[j + 1] == 1u ? 1 : 0;
+        std::vector<uint32_t> idx;
+        std::vector<uint8_t> val;
+        idx.reserve(last.size());
+        val.reserve(last.size());
+        for (const auto &kv : last) {
+            idx.push_back(kv.first);
+            val.push_back(kv.second);
+        }
+        g->flush();
+        uint32_t *d_idx = dalloc<uint32_t>(idx.size());
+        uint8_t *d_val = dalloc<uint8_t>(val.size());
+        hipError_t e1 = hipMemcpyAsync(d_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, g->stream);
+        hipError_t e2 = hipMemcpyAsync(d_val, val.data(), val.size(), hipMemcpyHostToDevice, g->stream);
+        if (e1 == hipSuccess && e2 == hipSuccess) {
+            hipLaunchKernelGGL(k_paint, dim3((unsigned)((idx.size() + 255) / 256)), dim3(256), 0, g->stream, d_idx,
+                               d_val, idx.size(), g->mask, (int)g->X, g->geom.pitch);
+            e1 = hipGetLastError();
+        }
+        hipError_t e3 = hipStreamSynchronize(g->stream);
+        (void)hipFree(d_idx);
+        (void)hipFree(d_val);
+        HIPCHK(e1);
+        HIPCHK(e2);
+        HIPCHK(e3);
+        g->build_flags();
+    });
+}
+
+int lbm_set_mask(lbm_grid *g, const uint8_t *solid) {
+    return guarded(g, [&] {
+        check(solid != nullptr, "null mask");
+        std::vector<uint8_t> m((size_t)g->X * g->Y);
+        for (size_t i = 0; i < m.size(); ++i) m[i] = solid[i] == 1 ? 1 : 0;
+        g->flush();
+        HIPCHK(hipMemcpy2DAsync(g->mask, g->geom.pitch, m.data(), g->X, g->X, g->Y, hipMemcpyHostToDevice,
+                                g->stream));
+        g->build_flags();
+        HIPCHK(hipStreamSynchronize(g->stream));
+    });
+}
+
+int lbm_get_mask(lbm_grid *g, uint8_t *solid) {
+    return guarded(g, [&] {
+        check(solid != nullptr, "null mask");
+        HIPCHK(hipMemcpy2DAsync(solid, g->X, g->mask, g->geom.pitch, g->X, g->Y, hipMemcpyDeviceToHost, g->stream));
+        HIPCHK(hipStreamSynchronize(g->stream));
+    });
+}
+
+int lbm_step(lbm_grid *g, uint32_t n) {
+    return guarded(g, [&] { g->step(n); });
+}
+
+int lbm_step_timed(lbm_grid *g, uint32_t n, float *ms) {
+    return guarded(g, [&] {
+        check(ms != nullptr, "null output");
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        hipError_t err = hipEventCreate(&e1);
+        if (err != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            throw HipError{err, "hipEventCreate"};
+        }
+        try {
+            HIPCHK(hipEventRecord(e0, g->stream));
+            g->step(n);
+            HIPCHK(hipEventRecord(e1, g->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(ms, e0, e1));
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
+int lbm_sync(lbm_grid *g) {
+    return guarded(g, [&] { HIPCHK(hipStreamSynchronize(g->stream)); });
+}
+
+int lbm_step_count(const lbm_grid *g, uint64_t *n) {
+    if (!g || !n) return LBM_EINVAL;
+    *n = g->steps;
+    return LBM_OK;
+}
+
+int lbm_get_stream(const lbm_grid *g, void **stream) {
+    if (!g || !stream) return LBM_EINVAL;
+    *stream = (void *)g->stream;
+    return LBM_OK;
+}
+
+int lbm_read_moments(lbm_grid *g, float *jx, float *jy, float *rho, int lagged) {
+    return guarded(g, [&] {
+        const float *src = g->lag;
+        if (!lagged) {
+            g->moments_now(false);
+            src = g->now;
+        }
+        const size_t ps = g->geom.pstride;
+        if (jx) g->copy_f32_plane_out(src, jx);
+        if (jy) g->copy_f32_plane_out(src + ps, jy);
+        if (rho) g->copy_f32_plane_out(src + 2 * ps, rho);
+        HIPCHK(hipStreamSynchronize(g->stream));
+    });
+}
+
+int lbm_read_fields(lbm_grid *g, float *rho, float *ux, float *uy, float *curl, int lagged) {
+    return guarded(g, [&] {
+        const float *src = g->lag;
+        if (!lagged) {
+            g->moments_now(false);
+            src = g->now;
+        }
+        g->ensure_dense();
+        hipLaunchKernelGGL(k_fields, dim3((g->X + 255) / 256, g->Y), dim3(256), 0, g->stream, src, g->dense, g->geom);
+        HIPCHK(hipGetLastError());
+        const size_t dn = (size_t)g->X * g->Y;
+        float *dst[4] = {rho, ux, uy, curl};
+        for (int i = 0; i < 4; ++i)
+            if (dst[i])
+                HIPCHK(hipMemcpyAsync(dst[i], g->dense + i * dn, sizeof(float) * dn, hipMemcpyDeviceToHost, g->stream));
+        HIPCHK(hipStreamSynchronize(g->stream));
+    });
+}
+
+int lbm_summary(lbm_grid *g, int kind, float *out) {
+    return guarded(g, [&] {
+        check(kind >= 0 && kind <= 4, "unknown summary kind");
+        hipLaunchKernelGGL(k_summary, dim3((g->X + 255) / 256, g->Y), dim3(256), 0, g->stream, g->lag, g->out, g->geom,
+                           kind);
+        HIPCHK(hipGetLastError());
+        if (out) g->copy_f32_plane_out(g->out, out);
+        HIPCHK(hipStreamSynchronize(g->stream));
+    });
+}
+
+int lbm_conservation(lbm_grid *g, double out[3]) {
+    int rc = guarded(g, [&] {
+        check(out != nullptr, "null output");
+        g->moments_now(true);
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, g->stream, g->partials, g->npartials, g->sums);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, g->sums, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        HIPCHK(hipStreamSynchronize(g->stream));
+    });
+    if (rc == LBM_OK && !(std::isfinite(out[0]) && std::isfinite(out[1]) && std::isfinite(out[2])))
+        return fail(g, LBM_ENONFINITE, "non-finite mass or momentum");
+    return rc;
+}
+
+int lbm_read_f(lbm_grid *g, float *f9) {
+    return guarded(g, [&] {
+        check(f9 != nullptr, "null output");
+        g->flush();
+        const size_t dn = (size_t)g->X * g->Y;
+        for (int k = 0; k < 9; ++k) g->copy_plane_out(g->plane(g->cur, k), f9 + k * dn, k);
+    });
+}
+
+int lbm_write_f(lbm_grid *g, const float *f9) {
+    return guarded(g, [&] {
+        check(f9 != nullptr, "null input");
+        g->pform = false;
+        const size_t dn = (size_t)g->X * g->Y;
+        for (int s = 0; s < 2; ++s)
+            for (int k = 0; k < 9; ++k) g->copy_plane_in(f9 + k * dn, g->plane(s, k), k);
+    });
+}
+
+// checkpoint image: header, both sets, mask, fields, summary output
+namespace {
+struct StateHeader {
+    char magic[8];
+    uint32_t X, Y;
+    int32_t precision, boundary;
+    float omega;
+    int32_t cur, pform;
+    uint32_t pitch;
+    long long pstride;
+    uint64_t steps;
+};
+}  // namespace
+
+int lbm_state_size(const lbm_grid *g, size_t *bytes) {
+    if (!g || !bytes) return LBM_EINVAL;
+    const size_t ps = (size_t)g->geom.pstride;
+    *bytes = sizeof(StateHeader) + 2 * g->set_bytes() + ps + 3 * ps * sizeof(float) + ps * sizeof(float);
+    return LBM_OK;
+}
+
+int lbm_save_state(lbm_grid *g, void *buf, size_t bytes) {
+    return guarded(g, [&] {
+        size_t need = 0;
+        lbm_state_size(g, &need);
+        check(buf != nullptr && bytes >= need, "buffer too small");
+        StateHeader h{};
+        std::memcpy(h.magic, "LBMHIP01", 8);
+        h.X = g->X;
+        h.Y = g->Y;
+        h.precision = g->precision;
+        h.boundary = g->boundary;
+        h.omega = g->omega;
+        h.cur = g->cur;
+        h.pform = g->pform;
+        h.pitch = (uint32_t)g->geom.pitch;
+        h.pstride = g->geom.pstride;
+        h.steps = g->steps;
+        char *p = (char *)buf;
+        std::memcpy(p, &h, sizeof h);
+        p += sizeof h;
+        const size_t ps = (size_t)g->geom.pstride;
+        HIPCHK(hipMemcpyAsync(p, g->f[0], g->set_bytes(), hipMemcpyDeviceToHost, g->stream));
+        p += g->set_bytes();
+        HIPCHK(hipMemcpyAsync(p, g->f[1], g->set_bytes(), hipMemcpyDeviceToHost, g->stream));
+        p += g->set_bytes();
+        HIPCHK(hipMemcpyAsync(p, g->mask, ps, hipMemcpyDeviceToHost, g->stream));
+        p += ps;
+        HIPCHK(hipMemcpyAsync(p, g->lag, 3 * ps * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+        p += 3 * ps * sizeof(float);
+        HIPCHK(hipMemcpyAsync(p, g->out, ps * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+        HIPCHK(hipStreamSynchronize(g->stream));
+    });
+}
+
+int lbm_load_state(lbm_grid *g, const void *buf, size_t bytes) {
+    return guarded(g, [&] {
+        size_t need = 0;
+        lbm_state_size(g, &need);
+        check(buf != nullptr && bytes >= need, "buffer too small");
+        StateHeader h;
+        std::memcpy(&h, buf, sizeof h);
+        check(std::memcmp(h.magic, "LBMHIP01", 8) == 0, "not a state image");
+        check(h.X == g->X && h.Y == g->Y && h.precision == g->precision && h.boundary == g->boundary &&
+                  (int)h.pitch == g->geom.pitch && h.pstride == g->geom.pstride,
+              "state image does not match this grid");
+        const char *p = (const char *)buf + sizeof h;
+        const size_t ps = (size_t)g->geom.pstride;
+        HIPCHK(hipMemcpyAsync(g->f[0], p, g->set_bytes(), hipMemcpyHostToDevice, g->stream));
+        p += g->set_bytes();
+        HIPCHK(hipMemcpyAsync(g->f[1], p, g->set_bytes(), hipMemcpyHostToDevice, g->stream));
+        p += g->set_bytes();
+        HIPCHK(hipMemcpyAsync(g->mask, p, ps, hipMemcpyHostToDevice, g->stream));
+        p += ps;
+        HIPCHK(hipMemcpyAsync(g->lag, p, 3 * ps * sizeof(float), hipMemcpyHostToDevice, g->stream));
+        p += 3 * ps * sizeof(float);
+        HIPCHK(hipMemcpyAsync(g->out, p, ps * sizeof(float), hipMemcpyHostToDevice, g->stream));
+        g->omega = h.omega;
+        g->geom.omega = h.omega;
+        g->cur = h.cur;
+        g->pform = h.pform != 0;
+        g->steps = h.steps;
+        g->build_flags();
+        HIPCHK(hipStreamSynchronize(g->stream));
+    });
+}
+
+// ---- generator -------------------------------------------------------------------
+
+uint64_t lbm_splitmix64(uint64_t *state) {
+    uint64_t z = (*state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static uint32_t draw_below(uint64_t *s, uint32_t m) {
+    return (uint32_t)(((lbm_splitmix64(s) >> 32) * (uint64_t)m) >> 32);
+}
+
+int lbm_gen_circles(uint8_t *solid, uint32_t nx, uint32_t ny, uint64_t seed, uint32_t n_circles, uint32_t r_min,
+                    uint32_t r_max) {
+    if (!solid || nx < 3 || ny < 3 || r_min > r_max || r_max + 2 > nx) return LBM_EINVAL;
+    uint64_t s = seed;
+    for (uint32_t c = 0; c < n_circles; ++c) {
+        const int64_t r = r_min + draw_below(&s, r_max - r_min + 1);
+        const int64_t cxv = r + 1 + draw_below(&s, nx - (uint32_t)r - 1);
+        const int64_t cyv = draw_below(&s, ny);
+        const int64_t y0 = std::max<int64_t>(0, cyv - r), y1 = std::min<int64_t>(ny - 1, cyv + r);
+        const int64_t x0 = std::max<int64_t>(0, cxv - r), x1 = std::min<int64_t>(nx - 1, cxv + r);
+        for (int64_t y = y0; y <= y1; ++y)
+            for (int64_t x = x0; x <= x1; ++x)
+                if ((x - cxv) * (x - cxv) + (y - cyv) * (y - cyv) <= r * r) solid[(size_t)y * nx + x] = 1;
+    }
+    return LBM_OK;
+}
+
+}  // extern "C"

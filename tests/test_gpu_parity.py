@@ -1,0 +1,337 @@
+"""GPU parity: the HIP step (through the C ABI) against the oracle.
+
+FP32 storage must equal refcpu (the faithful f32 restatement of the reference's
+eight passes) BIT FOR BIT on every cell of the state, on the renderer's fields
+and on the summary planes.  FP16-shifted storage and the zero-gradient outlet
+are compared with oracle64 (FP64) at stated tolerances.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle.np_ref import NpRef  # noqa: F401  (imported for parity with the CPU tests)
+
+pytestmark = pytest.mark.gpu
+
+X1, Y1 = 400, 160
+OMEGA1 = 1.0 / (3 * 0.02 + 0.5)  # config 1: nu = 0.02 (tau 0.56)
+U1 = 0.1
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def same_bits(a, b):
+    return np.array_equal(bits(a), bits(b))
+
+
+def circle_mask(X, Y, circles):
+    yy, xx = np.mgrid[0:Y, 0:X]
+    m = np.zeros((Y, X), np.uint8)
+    m[0, :] = m[-1, :] = 1
+    for cx, cy, r in circles:
+        m[(xx - cx) ** 2 + (yy - cy) ** 2 <= r * r] = 1
+    return m
+
+
+def make_pair(X, Y, omega, mask=None, u=U1, boundary=0, precision=0):
+    from lbm_amd import LBM
+    g = LBM(omega, X, Y, precision=precision, boundary=boundary)
+    r = oracle.RefCPU(X, Y, omega)
+    if mask is not None:
+        g.mask = mask
+        r.set_mask(mask.astype(np.uint32))
+    g.custom_speed(u)
+    r.reset_equilibrium(1.0, u, 0.0)
+    return g, r
+
+
+def assert_state_equal(g, r, what=""):
+    gs = g.read_f()
+    rs = r.state()
+    if not same_bits(gs, rs):
+        d = np.argwhere(bits(gs) != bits(rs))
+        pytest.fail(f"{what}: {len(d)} differing values, first at (k,y,x)={tuple(d[0])}: "
+                    f"{gs[tuple(d[0])]!r} vs {rs[tuple(d[0])]!r}")
+
+
+def assert_lagged_equal(g, r, what=""):
+    jx, jy, rho = g.moments(lagged=True)
+    rjx, rjy, rrho = r.density()
+    assert same_bits(jx, rjx), what
+    assert same_bits(jy, rjy), what
+    assert same_bits(rho, rrho), what
+
+
+def test_config1_bit_exact_2000_steps():
+    """Config 1 (400x160, u 0.1, nu 0.02, circle r=16 at (80,80)), 2000 steps."""
+    print("case: config1 400x160 2000 steps f32", flush=True)
+    mask = circle_mask(X1, Y1, [(80, 80, 16)])
+    g, r = make_pair(X1, Y1, OMEGA1, mask)
+    for n in (1, 9, 90, 900, 1000):
+        g.step(n)
+        r.step(n)
+        assert g.get_compute_num() == r.compute_step
+        assert_lagged_equal(g, r, f"lagged fields after {r.compute_step}")
+        assert_state_equal(g, r, f"state after {r.compute_step}")
+    for kind in range(5):
+        assert same_bits(g.calculate_summary(kind), r.summary(kind)), kind
+    # and against FP64: the error is reported, at the f32 floor
+    o = oracle.Oracle64(X1, Y1, OMEGA1, U1, 0)
+    o.set_solid(mask)
+    o.step(2000)
+    rho, ux, uy = o.macro()
+    grho, gux, guy, _ = g.fields()
+    fl = mask == 0
+    e_u = np.sqrt((((gux - ux) ** 2 + (guy - uy) ** 2)[fl]).sum() / ((ux ** 2 + uy ** 2)[fl]).sum())
+    print(f"config1 f32 vs fp64: rel-L2(u) = {e_u:.3e}")
+    assert e_u < 1e-4
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_random_geometry_paint_and_pulses(seed):
+    """Random small grids: random obstacles (including mask values other than 0/1),
+    painting between steps, omega changes, pulses, both sets differing."""
+    rng = np.random.default_rng(100 + seed)
+    X, Y = int(rng.integers(5, 90)), int(rng.integers(4, 50))
+    print(f"case: random seed={seed} {X}x{Y}", flush=True)
+    omega = float(rng.uniform(0.6, 1.95))
+    circles = [(int(rng.integers(1, X)), int(rng.integers(0, Y)), int(rng.integers(1, 5))) for _ in range(3)]
+    mask = circle_mask(X, Y, circles)
+    g, r = make_pair(X, Y, omega, mask, u=float(rng.uniform(0.0, 0.1)))
+    assert_lagged_equal(g, r, "after reset (pre-collision moments without rest)")
+    for it in range(6):
+        n = int(rng.integers(1, 25))
+        g.step(n)
+        r.step(n)
+        assert_state_equal(g, r, f"it {it}")
+        assert_lagged_equal(g, r, f"it {it}")
+        # paint a few cells, with repeats (last writer wins) and values 0/1/2
+        cells = rng.integers(0, X * Y, 6)
+        vals = rng.integers(0, 3, 6)
+        pairs = np.stack([np.concatenate([cells, cells[:2]]), np.concatenate([vals, 1 - np.minimum(vals[:2], 1)])],
+                         axis=1).ravel().astype(np.uint32)
+        g.paint(pairs)
+        assert r.paint(pairs) == 0
+        assert np.array_equal(g.mask, (r.mask() == 1).astype(np.uint8))
+        if it % 2:
+            omega = float(rng.uniform(0.6, 1.95))
+            g.update_omega_buffer(omega)
+            r.set_omega(omega)
+    g.step(5)
+    r.step(5)
+    assert_state_equal(g, r, "final")
+    for kind in range(5):
+        assert same_bits(g.calculate_summary(kind), r.summary(kind)), kind
+
+
+@pytest.mark.parametrize("k", list(range(9)))
+def test_single_cell_presets(k):
+    """single_cell(k) pulses (lbm.rs:1486-1496) then steps; the fields are not
+    reset by single_cell (no pre-collision pass)."""
+    print(f"case: single_cell k={k}", flush=True)
+    X, Y = 48, 20
+    g, r = make_pair(X, Y, 1.25, None)
+    g.step(3)
+    r.step(3)
+    g.single_cell(k)
+    r.single_cell(k)
+    assert g.get_compute_num() == 0
+    assert_lagged_equal(g, r, "fields survive single_cell")
+    assert_state_equal(g, r, "after single_cell")
+    g.step(17)
+    r.step(17)
+    assert_state_equal(g, r, "after 17 steps")
+    assert_lagged_equal(g, r, "after 17 steps")
+
+
+def test_pure_streaming_stream_demo():
+    """StreamDemo (tutorial.ts:129-134): nu huge -> omega ~ 0; an N pulse of 4.0 at
+    (3X/4, Y-2) travels north one row per step."""
+    X, Y = 40, 24
+    nu = 1e7
+    omega = np.float32(1.0) / (np.float32(3.0) * np.float32(nu) + np.float32(0.5))
+    g, r = make_pair(X, Y, float(omega), None)
+    g.update_viscosity(nu)
+    assert abs(g.omega - omega) == 0
+    g.single_cell(1)
+    r.single_cell(1)
+    r.set_omega(float(omega))
+    for s in range(1, 8):
+        g.step(1)
+        r.step(1)
+        assert_state_equal(g, r, f"stream demo step {s}")
+    f = g.read_f()
+    assert abs(f[1, Y - 2 - 7, 3 * X // 4] - 4.0) < 1e-5
+
+
+def test_zero_omega_is_a_permutation():
+    X, Y = 23, 11
+    rng = np.random.default_rng(5)
+    mask = circle_mask(X, Y, [(10, 5, 2)])
+    g, r = make_pair(X, Y, 0.0, mask)
+    f = (oracle.host_feq(0.05, 0.0, 1.0)[:, None, None] * (1 + 0.3 * rng.random((9, Y, X)))).astype(np.float32)
+    g.write_f(f)
+    r.set_set(0, f)
+    r.set_set(1, f)
+    r.compute_step = 0
+    g_count = g.get_compute_num()
+    assert g_count == 0
+    for s in range(12):
+        g.step(1)
+        r.step(1)
+    assert_state_equal(g, r, "omega 0")
+
+
+def test_step_splitting_and_checkpoint_continuation():
+    """N+N steps == 2N steps; save after N, load into a fresh grid, N more: same bits."""
+    print("case: checkpoint 200x80", flush=True)
+    from lbm_amd import LBM
+    X, Y = 200, 80
+    mask = circle_mask(X, Y, [(40, 40, 9), (120, 30, 6)])
+    a, _ = make_pair(X, Y, 1.9, mask)
+    b, _ = make_pair(X, Y, 1.9, mask)
+    a.step(150)
+    for _ in range(150):
+        b.step(1)
+    assert same_bits(a.read_f(), b.read_f())
+    blob = a.save_state()
+    a.step(150)
+    c = LBM(1.9, X, Y)
+    c.load_state(blob)
+    assert c.get_compute_num() == 150
+    c.step(150)
+    assert same_bits(a.read_f(), c.read_f())
+    assert same_bits(a.moments(True)[2], c.moments(True)[2])
+    # read_f between steps does not perturb the trajectory
+    d, _ = make_pair(X, Y, 1.9, mask)
+    for _ in range(3):
+        d.step(100)
+        d.read_f()
+    assert same_bits(a.read_f(), d.read_f())
+
+
+def test_conservation_reduction_matches_cpu_sum():
+    X, Y = 301, 77
+    rng = np.random.default_rng(9)
+    mask = circle_mask(X, Y, [(100, 40, 10)])
+    g, _ = make_pair(X, Y, 1.7, mask)
+    g.step(123)
+    jx, jy, rho = g.moments(lagged=False)
+    fl = g.mask != 1
+    ref = np.array([rho[fl].astype(np.float64).sum(), jx[fl].astype(np.float64).sum(),
+                    jy[fl].astype(np.float64).sum()])
+    got = g.conservation()
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-9)
+    # non-lagged moments are those of the current state (moments of read_f)
+    mjx, mjy, mrho = oracle.moments_ref_order(g.read_f())
+    assert same_bits(mrho, rho) and same_bits(mjx, jx) and same_bits(mjy, jy)
+    del rng
+
+
+def test_nonfinite_state_is_reported():
+    from lbm_amd import LbmError
+    g, _ = make_pair(64, 32, 1.2, None)
+    g.step(3)
+    g.set_pulse(5, 10, 10, float("nan"))
+    g.step(2)
+    with pytest.raises(LbmError) as e:
+        g.conservation()
+    assert e.value.code == -4
+
+
+def test_paint_errors_change_nothing():
+    from lbm_amd import LbmError
+    g, _ = make_pair(32, 16, 1.2, None)
+    m0 = g.mask.copy()
+    with pytest.raises(LbmError):
+        g.paint(np.array([5, 1, 7], np.uint32))  # odd length
+    with pytest.raises(LbmError):
+        g.paint(np.array([5, 1, 32 * 16, 1], np.uint32))  # out of range
+    g.paint(np.array([], np.uint32))  # empty: no-op
+    assert np.array_equal(g.mask, m0)
+
+
+def test_zero_gradient_outlet_vs_fp64():
+    print("case: zero-gradient 300x100", flush=True)
+    from lbm_amd import LBM, BOUNDARY_ZERO_GRADIENT
+    X, Y, omega, u = 300, 100, 1.0 / 0.56, 0.08
+    mask = circle_mask(X, Y, [(60, 50, 10)])
+    g = LBM(omega, X, Y, boundary=BOUNDARY_ZERO_GRADIENT)
+    g.mask = mask
+    g.custom_speed(u)
+    g.step(1500)
+    o = oracle.Oracle64(X, Y, omega, u, 1)
+    o.set_solid(mask)
+    o.set_equilibrium(1.0, u, 0.0)
+    o.step(1500)
+    rho, ux, uy = o.macro()
+    grho, gux, guy, _ = g.fields()
+    fl = mask == 0
+    e_u = np.sqrt((((gux - ux) ** 2 + (guy - uy) ** 2)[fl]).sum() / ((ux ** 2 + uy ** 2)[fl]).sum())
+    print(f"zero-gradient f32 vs fp64 rel-L2(u) = {e_u:.3e}")
+    assert e_u < 1e-4
+
+
+def test_f16s_codec_round_trip():
+    from lbm_amd import LBM, PRECISION_F16S
+    X, Y = 70, 30
+    rng = np.random.default_rng(1)
+    g = LBM(1.2, X, Y, precision=PRECISION_F16S)
+    w = np.array([1 / 36, 1 / 9, 1 / 36, 1 / 9, 4 / 9, 1 / 9, 1 / 36, 1 / 9, 1 / 36], np.float32)
+    f = (w[:, None, None] * (1 + 0.2 * rng.standard_normal((9, Y, X)))).astype(np.float32)
+    g.write_f(f)
+    got = g.read_f()
+    exp = (f - w[:, None, None]).astype(np.float16).astype(np.float32) + w[:, None, None]
+    assert same_bits(got, exp)
+
+
+@pytest.mark.parametrize("steps", [1000])
+def test_f16s_error_vs_fp64_config1(steps):
+    """FP16-shifted storage, FP32 collide: velocity error vs FP64 on config 1."""
+    from lbm_amd import LBM, PRECISION_F16S
+    mask = circle_mask(X1, Y1, [(80, 80, 16)])
+    g = LBM(OMEGA1, X1, Y1, precision=PRECISION_F16S)
+    g.mask = mask
+    g.custom_speed(U1)
+    g.step(steps)
+    o = oracle.Oracle64(X1, Y1, OMEGA1, U1, 0)
+    o.set_solid(mask)
+    o.step(steps)
+    rho, ux, uy = o.macro()
+    grho, gux, guy, _ = g.fields()
+    fl = mask == 0
+    e_u = np.sqrt((((gux - ux) ** 2 + (guy - uy) ** 2)[fl]).sum() / ((ux ** 2 + uy ** 2)[fl]).sum())
+    print(f"config1 f16s vs fp64 after {steps}: rel-L2(u) = {e_u:.3e}")
+    assert e_u < 1e-2
+
+
+def test_full_size_properties_8192x4096():
+    """Config 2 at its full size: properties that do not need the CPU oracle.
+    The inlet column stays the exact f32 inflow equilibrium, one lbm_step(n)
+    equals n lbm_step(1) calls bit for bit (checked through conservation sums and
+    a sampled window of the state), and mass stays finite."""
+    print("case: full size 8192x4096", flush=True)
+    from lbm_amd import LBM, gen_circles
+    X, Y, omega, u = 8192, 4096, 1.0 / 0.53, 0.08
+    mask = gen_circles(X, Y, 2024, 200, 8, 64)
+    a = LBM(omega, X, Y)
+    a.mask = mask
+    a.custom_speed(u)
+    a.step(40)
+    sa = a.conservation()
+    b = LBM(omega, X, Y)
+    b.mask = mask
+    b.custom_speed(u)
+    for _ in range(40):
+        b.step(1)
+    sb = b.conservation()
+    assert np.array_equal(sa, sb)
+    fa = a.read_f()
+    fb = b.read_f()
+    assert same_bits(fa[:, ::97, ::89], fb[:, ::97, ::89])
+    feq = oracle.host_feq(u, 0.0, 1.0)
+    col = fa[:, 1:-1, 0]
+    assert same_bits(col, np.broadcast_to(feq[:, None], col.shape))
+    assert np.isfinite(sa).all()
