@@ -46,11 +46,16 @@ def read_traffic(name):
     return float(m.group(1)) if m else None
 
 
-def run_gpu(precision, X, Y, mask, steps, warmup, omega, u_in):
+def run_gpu(precision, X, Y, mask, steps, warmup, omega, u_in, tune=True):
     from lbm_amd import LBM
     g = LBM(omega, X, Y, precision=precision)
     g.mask = mask
     g.custom_speed(u_in)
+    tuned = None
+    if tune:
+        best, ms = g.autotune(20)  # state is restored afterwards
+        tuned = {"variant": best, "us_per_step_by_variant": [round(m * 1e3, 2) for m in ms]}
+        log(f"autotune: variant {best}, us/step {tuned['us_per_step_by_variant']}")
     g.step(warmup)
     g.sync()
     t0 = time.perf_counter()
@@ -58,7 +63,7 @@ def run_gpu(precision, X, Y, mask, steps, warmup, omega, u_in):
     wall = time.perf_counter() - t0
     cons = g.conservation()
     g.close()
-    return wall, dev_ms, cons
+    return wall, dev_ms, cons, tuned
 
 
 def cpu_baseline(X, Y, mask, omega, u_in, budget_s):
@@ -116,7 +121,7 @@ def main():
     solid_frac = float(mask[1:-1].mean())
 
     log(f"f32: warmup {args.warmup}, timed {args.steps}")
-    wall, dev_ms, cons = run_gpu(PRECISION_F32, X, Y, mask, args.steps, args.warmup, omega, u_in)
+    wall, dev_ms, cons, tuned = run_gpu(PRECISION_F32, X, Y, mask, args.steps, args.warmup, omega, u_in)
     mlups = cells * args.steps / wall / 1e6
     kern_s = dev_ms / 1e3 / args.steps
     achieved = 72.0 * cells / kern_s / 1e9
@@ -153,11 +158,12 @@ def main():
             "kernel_avg_us": round(kern_s * 1e6, 2),
         },
         "fp64_mass_momentum": [float(c) for c in cons],
+        "autotune": tuned,
     }
 
     if not args.no_f16:
         log("f16s: same workload, FP16-shifted storage")
-        w16, d16, c16 = run_gpu(PRECISION_F16S, X, Y, mask, args.steps, args.warmup, omega, u_in)
+        w16, d16, c16, tuned16 = run_gpu(PRECISION_F16S, X, Y, mask, args.steps, args.warmup, omega, u_in)
         m16 = cells * args.steps / w16 / 1e6
         k16 = d16 / 1e3 / args.steps
         a16 = 36.0 * cells / k16 / 1e9
@@ -171,6 +177,7 @@ def main():
                          "traffic": read_traffic("traffic_f16s_8192x4096.txt") if (X, Y) == (8192, 4096)
                          else None},
             "fp64_mass_momentum": [float(c) for c in c16],
+            "autotune": tuned16,
         }
 
     if not args.no_cpu_baseline:

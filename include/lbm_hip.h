@@ -132,6 +132,17 @@ int lbm_get_mask(lbm_grid *g, uint8_t *solid);
 int lbm_step(lbm_grid *g, uint32_t n);
 /* Blocks until the grid's stream is idle. */
 int lbm_sync(lbm_grid *g);
+
+/* Step-kernel variants (no reference counterpart): how the fused step moves
+ * memory -- cells per thread and non-temporal access.  All variants compute the
+ * same bits.  lbm_autotune times each for `steps` steps on the grid's own
+ * state, restores that state exactly, selects the fastest, and returns it in
+ * *best and the per-step time of every variant in ms[LBM_NUM_VARIANTS] (either
+ * pointer may be NULL). */
+#define LBM_NUM_VARIANTS 4
+int lbm_set_variant(lbm_grid *g, int variant);
+int lbm_get_variant(const lbm_grid *g, int *variant);
+int lbm_autotune(lbm_grid *g, uint32_t steps, int *best, float *ms);
 /* lbm_step(g, n) bracketed by two hipEvents recorded on the grid's stream;
  * waits for the second and returns the elapsed device time in *ms.  (No
  * reference counterpart: the reference has no timing, lbm.rs:1166-1172.) */

@@ -18,6 +18,8 @@ LBM_ENOMEM = -3
 LBM_ENONFINITE = -4
 LBM_ENODEV = -5
 
+NUM_VARIANTS = 4
+
 PRECISION_F32 = 0
 PRECISION_F16S = 1
 BOUNDARY_REFERENCE = 0
@@ -71,6 +73,9 @@ SIGNATURES = {
     "lbm_step": (_i32, [_P, _u32]),
     "lbm_sync": (_i32, [_P]),
     "lbm_step_timed": (_i32, [_P, _u32, _fp]),
+    "lbm_set_variant": (_i32, [_P, _i32]),
+    "lbm_get_variant": (_i32, [_P, ctypes.POINTER(ctypes.c_int)]),
+    "lbm_autotune": (_i32, [_P, _u32, ctypes.POINTER(ctypes.c_int), _fp]),
     "lbm_step_count": (_i32, [_P, _u64p]),
     "lbm_get_stream": (_i32, [_P, _PP]),
     "lbm_read_moments": (_i32, [_P, _fp, _fp, _fp, _i32]),

@@ -112,86 +112,196 @@ enum : int {
 
 constexpr int kBlock = 256;
 
-template <class St, int MODE, bool PFORM, int OP>
+// CPT consecutive cells of a row per thread, moved as one vector: aligned for the
+// cell's own planes, element-aligned for the planes pulled from x - 1 / x + 1
+template <class T, int N>
+struct Vec {
+    typedef T a __attribute__((ext_vector_type(N)));
+    typedef T u __attribute__((ext_vector_type(N), aligned(sizeof(T))));
+};
+
+template <class V, bool NT>
+__device__ __forceinline__ V vload(const V *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <class V, bool NT>
+__device__ __forceinline__ void vstore(V v, V *p) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// Workgroup ids are handed to the eight XCDs in turn; remap them so that each
+// XCD walks one contiguous band of rows.  Neighbouring row segments, which
+// share the cache lines of their x - 1 / x + 1 pulls, then meet in one L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    const int q = n >> 3;
+    if (b >= (q << 3)) return b;
+    return (b & 7) * q + (b >> 3);
+}
+
+// The state S of one cell before collision.  P-form: pull from src with the
+// bounce-back byte, or, for cells the stream does not write, keep dst's own
+// values; the rest population is src's.  S-form: src's own values.
+template <class St, int MODE, bool PFORM>
+__device__ __forceinline__ bool gather_cell(const typename St::T *src, const typename St::T *dst,
+                                            const uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask,
+                                            const Geom &g, int x, int y, size_t own, float f[9]) {
+    const long long ps = g.pstride;
+    bool streamed = false;
+    if (PFORM) {
+        const uint8_t fl = flags[own];
+        if (fl != kSlowCell) {
+            streamed = true;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                if (k == C) continue;
+                int ux, uy;
+                upstream<MODE>(k, x, y, g.X, ux, uy);
+                const bool bounce = (fl >> bit(k)) & 1u;
+                const size_t idx = bounce ? (size_t)opp(k) * ps + own : (size_t)k * ps + (size_t)uy * g.pitch + ux;
+                f[k] = St::decode(src[idx], bounce ? opp(k) : k);
+            }
+        } else {
+            // enclosed streamed cell: every population bounces back; otherwise
+            // the cell is not streamed and keeps the other set's values
+            streamed = mask[own] != 1 && x != 0 && y < g.Y - 1;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                if (k == C) continue;
+                f[k] = streamed ? St::decode(src[(size_t)opp(k) * ps + own], opp(k))
+                                : St::decode(dst[(size_t)k * ps + own], k);
+            }
+        }
+        f[C] = St::decode(src[(size_t)C * ps + own], C);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f[k] = St::decode(src[(size_t)k * ps + own], k);
+    }
+    return streamed;
+}
+
+template <class St, int MODE, bool PFORM, int OP, int CPT, bool NT>
 __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, typename St::T *dst,
                                                  const uint8_t *__restrict__ flags,
                                                  const uint8_t *__restrict__ mask, Geom g,
-                                                 float *__restrict__ mom, double *__restrict__ partials) {
+                                                 float *__restrict__ mom, double *__restrict__ partials,
+                                                 int blocks_per_row, int nblocks) {
     using T = typename St::T;
-    const int x = blockIdx.x * kBlock + threadIdx.x;
-    const int y = blockIdx.y;
-    const bool valid = x < g.X;
-    if (!(OP & OP_REDUCE) && !valid) return;
-    const size_t own = (size_t)y * g.pitch + (valid ? x : 0);
+    using V = typename Vec<T, CPT>::a;
+    using U = typename Vec<T, CPT>::u;
+    using VF = typename Vec<float, CPT>::a;
+    const int L = xcd_remap(blockIdx.x, nblocks);
+    const int y = L / blocks_per_row;
+    const int x0 = ((L - y * blocks_per_row) * kBlock + threadIdx.x) * CPT;
     const long long ps = g.pstride;
+    const size_t own0 = (size_t)y * g.pitch + x0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 
-    float f[9];
-    bool streamed = false;
-    if (valid) {
-        if (PFORM) {
-            const uint8_t fl = flags[own];
-            if (fl != kSlowCell) {
-                streamed = true;
+    if (x0 < g.X) {
+        // fast path: the whole group lies before the outlet column and (P-form)
+        // every cell is streamed with no bounce-back
+        bool fast = x0 + CPT < g.X;
+        if (PFORM && fast) {
+            if constexpr (CPT == 8) fast = *(const uint64_t *)(flags + own0) == 0;
+            else if constexpr (CPT == 4) fast = *(const uint32_t *)(flags + own0) == 0;
+            else fast = *(const uint16_t *)(flags + own0) == 0;
+        }
+        if (fast) {
+            V v[9];
 #pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    if (k == C) continue;
-                    if (fl & (1u << bit(k))) {
-                        f[k] = St::decode(src[opp(k) * ps + own], opp(k));
-                    } else {
-                        int ux, uy;
-                        upstream<MODE>(k, x, y, g.X, ux, uy);
-                        f[k] = St::decode(src[k * ps + (size_t)uy * g.pitch + ux], k);
-                    }
-                }
-            } else {
-                streamed = mask[own] != 1 && x != 0 && y < g.Y - 1;
-#pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    if (k == C) continue;
-                    // enclosed streamed cell: every population bounces back;
-                    // otherwise the cell is not streamed and keeps the other set
-                    f[k] = streamed ? St::decode(src[opp(k) * ps + own], opp(k))
-                                    : St::decode(dst[k * ps + own], k);
+            for (int k = 0; k < 9; ++k) {
+                if (PFORM) {
+                    const size_t idx = (size_t)(y - cy(k)) * g.pitch + (x0 - cx(k));
+                    if (cx(k) == 0) v[k] = vload<V, NT>((const V *)(src + (size_t)k * ps + idx));
+                    else v[k] = (V)vload<U, NT>((const U *)(src + (size_t)k * ps + idx));
+                } else {
+                    v[k] = vload<V, NT>((const V *)(src + (size_t)k * ps + own0));
                 }
             }
-            f[C] = St::decode(src[C * ps + own], C);
+            V o[9];
+            VF mjx, mjy, mrho;
+            uint64_t fluid_bits = ~0ull;
+            if ((OP & OP_REDUCE) && !PFORM) {
+                fluid_bits = 0;
+#pragma unroll
+                for (int j = 0; j < CPT; ++j) fluid_bits |= (uint64_t)(mask[own0 + j] != 1) << j;
+            }
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) {
+                float f[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) f[k] = St::decode(v[k][j], k);
+                Moments m;
+                if (OP & OP_COLLIDE) {
+                    m = collide_ref(f, g.omega);
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) o[k][j] = St::encode(f[k], k);
+                } else {
+                    m = moments_ref(f, !(OP & OP_NOREST));
+                }
+                mjx[j] = m.jx;
+                mjy[j] = m.jy;
+                mrho[j] = m.rho;
+                if ((OP & OP_REDUCE) && ((fluid_bits >> j) & 1)) {
+                    s0 += (double)m.rho;
+                    s1 += (double)m.jx;
+                    s2 += (double)m.jy;
+                }
+            }
+            if (OP & OP_COLLIDE) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) vstore<V, NT>(o[k], (V *)(dst + (size_t)k * ps + own0));
+            }
+            if (OP & OP_STORE_S) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) vstore<V, NT>(v[k], (V *)(dst + (size_t)k * ps + own0));
+            }
+            if (OP & OP_MOMENTS) {
+                *(VF *)(mom + own0) = mjx;
+                *(VF *)(mom + ps + own0) = mjy;
+                *(VF *)(mom + 2 * ps + own0) = mrho;
+            }
         } else {
+            // general path, one cell at a time
+            for (int j = 0; j < CPT; ++j) {
+                const int x = x0 + j;
+                if (x >= g.X) break;
+                const size_t own = own0 + j;
+                float f[9];
+                const bool streamed = gather_cell<St, MODE, PFORM>(src, dst, flags, mask, g, x, y, own, f);
+                Moments m;
+                if (OP & OP_STORE_S) {
+                    if (streamed) {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) f[k] = St::decode(src[k * ps + own], k);
+                        for (int k = 0; k < 9; ++k)
+                            if (k != C) dst[(size_t)k * ps + own] = St::encode(f[k], k);
+                    }
+                    dst[(size_t)C * ps + own] = src[(size_t)C * ps + own];
+                }
+                if (OP & OP_COLLIDE) {
+                    m = collide_ref(f, g.omega);
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) dst[(size_t)k * ps + own] = St::encode(f[k], k);
+                } else {
+                    m = moments_ref(f, !(OP & OP_NOREST));
+                }
+                if (OP & OP_MOMENTS) {
+                    mom[own] = m.jx;
+                    mom[ps + own] = m.jy;
+                    mom[2 * ps + own] = m.rho;
+                }
+                if ((OP & OP_REDUCE) && mask[own] != 1) {
+                    s0 += (double)m.rho;
+                    s1 += (double)m.jx;
+                    s2 += (double)m.jy;
+                }
+            }
         }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) f[k] = 0.0f;
     }
 
-    Moments m;
-    if (OP & OP_STORE_S) {
-        if (streamed) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k)
-                if (k != C) dst[k * ps + own] = St::encode(f[k], k);
-        }
-        dst[C * ps + own] = src[C * ps + own];
-    }
-    if (OP & OP_COLLIDE) {
-        m = collide_ref(f, g.omega);
-#pragma unroll
-        for (int k = 0; k < 9; ++k) dst[k * ps + own] = St::encode(f[k], k);
-    } else {
-        m = moments_ref(f, !(OP & OP_NOREST));
-    }
-    if (OP & OP_MOMENTS) {
-        if (valid) {
-            mom[own] = m.jx;
-            mom[ps + own] = m.jy;
-            mom[2 * ps + own] = m.rho;
-        }
-    }
     if (OP & OP_REDUCE) {
-        const bool fluid = valid && mask[own] != 1;
-        double s0 = fluid ? (double)m.rho : 0.0;
-        double s1 = fluid ? (double)m.jx : 0.0;
-        double s2 = fluid ? (double)m.jy : 0.0;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             s0 += __shfl_xor(s0, off, 64);
@@ -209,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, type
         if (threadIdx.x < 3) {
             double t = 0.0;
             for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
-            partials[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = t;
+            partials[(size_t)blockIdx.x * 3 + threadIdx.x] = t;
         }
     }
 }
@@ -389,11 +499,19 @@ struct lbm_grid {
     double *sums = nullptr;
     size_t npartials = 0;
     uint64_t steps = 0;
+    int variant = 0;  // step-kernel variant (cells per thread, non-temporal access)
     std::string err;
 
     size_t plane_bytes() const { return (size_t)geom.pstride * esize; }
     size_t set_bytes() const { return 9 * plane_bytes(); }
-    dim3 cell_grid() const { return dim3((X + kBlock - 1) / kBlock, Y); }
+    struct Shape {
+        int blocks_per_row, nblocks;
+    };
+    Shape shape(int cpt) const {
+        const int groups = ((int)X + cpt - 1) / cpt;
+        const int bpr = (groups + kBlock - 1) / kBlock;
+        return {bpr, bpr * (int)Y};
+    }
 
     void release() {
         (void)hipSetDevice(dev);
@@ -408,25 +526,52 @@ struct lbm_grid {
         stream = nullptr;
     }
 
-    // ---- kernels launchers -------------------------------------------------
-    template <class St, bool PFORM, int OP>
-    void launch_cell(const void *src, void *dst, float *mom, double *part) {
+    // ---- kernel launchers --------------------------------------------------
+    template <class St, int MODE, bool PFORM, int OP, int CPT, bool NT>
+    void go(const void *src, void *dst, float *mom, double *part) {
         using T = typename St::T;
-        if (boundary == LBM_BOUNDARY_REFERENCE)
-            hipLaunchKernelGGL((k_cell<St, LBM_BOUNDARY_REFERENCE, PFORM, OP>), cell_grid(), dim3(kBlock), 0,
-                               stream, (const T *)src, (T *)dst, flags, mask, geom, mom, part);
-        else
-            hipLaunchKernelGGL((k_cell<St, LBM_BOUNDARY_ZERO_GRADIENT, PFORM, OP>), cell_grid(), dim3(kBlock), 0,
-                               stream, (const T *)src, (T *)dst, flags, mask, geom, mom, part);
+        const Shape sh = shape(CPT);
+        hipLaunchKernelGGL((k_cell<St, MODE, PFORM, OP, CPT, NT>), dim3(sh.nblocks), dim3(kBlock), 0, stream,
+                           (const T *)src, (T *)dst, flags, mask, geom, mom, part, sh.blocks_per_row, sh.nblocks);
         HIPCHK(hipGetLastError());
+    }
+
+    // default vector width: 16 bytes per plane access
+    static constexpr int default_cpt(int precision) { return precision == LBM_PRECISION_F32 ? 4 : 8; }
+
+    // The fused P-form step is the hot kernel and has tuned variants; every
+    // other pass runs variant 0.  Variants differ only in how memory is moved,
+    // never in arithmetic, so all of them give the same bits.
+    template <class St, int MODE, bool PFORM, int OP>
+    void launch_cell(const void *src, void *dst, float *mom, double *part) {
+        constexpr int C0 = sizeof(typename St::T) == 4 ? 4 : 8;
+        constexpr bool hot = PFORM && (OP & OP_COLLIDE);
+        const int v = hot ? variant : 0;
+        switch (v) {
+        case 1: if constexpr (hot) { go<St, MODE, PFORM, OP, C0, false>(src, dst, mom, part); return; }
+                break;
+        case 2: if constexpr (hot) { go<St, MODE, PFORM, OP, C0 / 2, true>(src, dst, mom, part); return; }
+                break;
+        case 3: if constexpr (hot) { go<St, MODE, PFORM, OP, (C0 == 4 ? 8 : 2), true>(src, dst, mom, part); return; }
+                break;
+        default: break;
+        }
+        go<St, MODE, PFORM, OP, C0, true>(src, dst, mom, part);
     }
 
     template <bool PFORM, int OP>
     void cell(const void *src, void *dst, float *mom, double *part) {
-        if (precision == LBM_PRECISION_F32)
-            launch_cell<StoreF32, PFORM, OP>(src, dst, mom, part);
-        else
-            launch_cell<StoreF16S, PFORM, OP>(src, dst, mom, part);
+        if (precision == LBM_PRECISION_F32) {
+            if (boundary == LBM_BOUNDARY_REFERENCE)
+                launch_cell<StoreF32, LBM_BOUNDARY_REFERENCE, PFORM, OP>(src, dst, mom, part);
+            else
+                launch_cell<StoreF32, LBM_BOUNDARY_ZERO_GRADIENT, PFORM, OP>(src, dst, mom, part);
+        } else {
+            if (boundary == LBM_BOUNDARY_REFERENCE)
+                launch_cell<StoreF16S, LBM_BOUNDARY_REFERENCE, PFORM, OP>(src, dst, mom, part);
+            else
+                launch_cell<StoreF16S, LBM_BOUNDARY_ZERO_GRADIENT, PFORM, OP>(src, dst, mom, part);
+        }
     }
 
     void build_flags() {
@@ -462,8 +607,7 @@ struct lbm_grid {
     }
     void ensure_partials() {
         if (!partials) {
-            const dim3 gr = cell_grid();
-            npartials = (size_t)gr.x * gr.y;
+            npartials = (size_t)shape(default_cpt(precision)).nblocks;
             partials = dalloc<double>(3 * npartials);
             sums = dalloc<double>(3);
         }
@@ -606,7 +750,7 @@ int lbm_create(const lbm_config *cfg, lbm_grid **out) {
     if (!cfg || !out) return fail(nullptr, LBM_EINVAL, "null argument");
     *out = nullptr;
     if (cfg->nx < 3 || cfg->ny < 3) return fail(nullptr, LBM_EINVAL, "nx and ny must be >= 3");
-    if (cfg->ny > 65535) return fail(nullptr, LBM_EINVAL, "ny must be <= 65535 (one launch row per grid row)");
+    if (cfg->row_pad % 8 != 0) return fail(nullptr, LBM_EINVAL, "row_pad must be a multiple of 8 elements");
     if ((uint64_t)cfg->nx * cfg->ny > 0xFFFFFFFFull)
         return fail(nullptr, LBM_EINVAL, "nx * ny must fit in 32 bits (the paint wire format is u32)");
     if (cfg->precision != LBM_PRECISION_F32 && cfg->precision != LBM_PRECISION_F16S)
@@ -850,6 +994,69 @@ int lbm_step_timed(lbm_grid *g, uint32_t n, float *ms) {
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+    });
+}
+
+int lbm_set_variant(lbm_grid *g, int variant) {
+    return guarded(g, [&] {
+        check(variant >= 0 && variant < LBM_NUM_VARIANTS, "unknown kernel variant");
+        g->variant = variant;
+    });
+}
+
+int lbm_get_variant(const lbm_grid *g, int *variant) {
+    if (!g || !variant) return LBM_EINVAL;
+    *variant = g->variant;
+    return LBM_OK;
+}
+
+int lbm_autotune(lbm_grid *g, uint32_t steps, int *best, float *ms) {
+    return guarded(g, [&] {
+        check(steps > 0, "steps must be > 0");
+        // snapshot the whole evolving state so that tuning leaves no trace
+        const size_t sb = g->set_bytes(), lb = 3 * (size_t)g->geom.pstride * sizeof(float);
+        char *snap = dalloc<char>(2 * sb + lb);
+        const int cur = g->cur;
+        const bool pform = g->pform;
+        const uint64_t nsteps = g->steps;
+        float t[LBM_NUM_VARIANTS];
+        int v_best = 0;
+        try {
+            HIPCHK(hipMemcpyAsync(snap, g->f[0], sb, hipMemcpyDeviceToDevice, g->stream));
+            HIPCHK(hipMemcpyAsync(snap + sb, g->f[1], sb, hipMemcpyDeviceToDevice, g->stream));
+            HIPCHK(hipMemcpyAsync(snap + 2 * sb, g->lag, lb, hipMemcpyDeviceToDevice, g->stream));
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            for (int v = 0; v < LBM_NUM_VARIANTS; ++v) {
+                g->variant = v;
+                g->step(2);  // warm-up (and leaves the grid in P-form)
+                HIPCHK(hipEventRecord(e0, g->stream));
+                g->step(steps);
+                HIPCHK(hipEventRecord(e1, g->stream));
+                HIPCHK(hipEventSynchronize(e1));
+                HIPCHK(hipEventElapsedTime(&t[v], e0, e1));
+                if (t[v] < t[v_best]) v_best = v;
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            HIPCHK(hipMemcpyAsync(g->f[0], snap, sb, hipMemcpyDeviceToDevice, g->stream));
+            HIPCHK(hipMemcpyAsync(g->f[1], snap + sb, sb, hipMemcpyDeviceToDevice, g->stream));
+            HIPCHK(hipMemcpyAsync(g->lag, snap + 2 * sb, lb, hipMemcpyDeviceToDevice, g->stream));
+            HIPCHK(hipStreamSynchronize(g->stream));
+        } catch (...) {
+            (void)hipStreamSynchronize(g->stream);
+            (void)hipFree(snap);
+            throw;
+        }
+        (void)hipFree(snap);
+        g->cur = cur;
+        g->pform = pform;
+        g->steps = nsteps;
+        g->variant = v_best;
+        if (best) *best = v_best;
+        if (ms)
+            for (int v = 0; v < LBM_NUM_VARIANTS; ++v) ms[v] = t[v] / steps;
     });
 }
 

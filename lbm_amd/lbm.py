@@ -172,6 +172,24 @@ class LBM:
         self._chk(self._lib.lbm_step_timed(self._h, n, ctypes.byref(ms)))
         return ms.value
 
+    @property
+    def variant(self) -> int:
+        v = ctypes.c_int()
+        self._chk(self._lib.lbm_get_variant(self._h, ctypes.byref(v)))
+        return v.value
+
+    @variant.setter
+    def variant(self, v: int):
+        self._chk(self._lib.lbm_set_variant(self._h, v))
+
+    def autotune(self, steps: int = 20):
+        """Time every step-kernel variant on this grid (state restored), keep the
+        fastest; returns (best, ms_per_step list)."""
+        best = ctypes.c_int()
+        ms = (ctypes.c_float * N.NUM_VARIANTS)()
+        self._chk(self._lib.lbm_autotune(self._h, steps, ctypes.byref(best), ms))
+        return best.value, list(ms)
+
     def sync(self):
         self._chk(self._lib.lbm_sync(self._h))
 

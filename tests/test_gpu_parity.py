@@ -335,3 +335,48 @@ def test_full_size_properties_8192x4096():
     col = fa[:, 1:-1, 0]
     assert same_bits(col, np.broadcast_to(feq[:, None], col.shape))
     assert np.isfinite(sa).all()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_every_kernel_variant_gives_the_same_bits(precision, variant):
+    """The step-kernel variants move memory differently and must not change a bit.
+    Odd width: exercises partial vector groups, the outlet column and the slow path."""
+    print(f"case: variant {variant} precision {precision}", flush=True)
+    from lbm_amd import LBM
+    X, Y = 203, 61
+    mask = circle_mask(X, Y, [(50, 30, 8), (140, 20, 5), (202, 40, 3)])
+    ref = LBM(1.8, X, Y, precision=precision)
+    ref.mask = mask
+    ref.custom_speed(0.09)
+    ref.variant = 0
+    ref.step(120)
+    g = LBM(1.8, X, Y, precision=precision)
+    g.mask = mask
+    g.custom_speed(0.09)
+    g.variant = variant
+    g.step(120)
+    assert same_bits(g.read_f(), ref.read_f())
+    if precision == 0:
+        r = oracle.RefCPU(X, Y, 1.8)
+        r.set_mask(mask.astype(np.uint32))
+        r.reset_equilibrium(1.0, 0.09, 0.0)
+        r.step(120)
+        assert_state_equal(g, r, f"variant {variant}")
+
+
+def test_autotune_leaves_the_state_untouched():
+    from lbm_amd import LBM
+    X, Y = 640, 200
+    mask = circle_mask(X, Y, [(100, 100, 20)])
+    a, _ = make_pair(X, Y, 1.7, mask)
+    b, _ = make_pair(X, Y, 1.7, mask)
+    a.step(33)
+    b.step(33)
+    best, ms = a.autotune(5)
+    assert 0 <= best < 4 and len(ms) == 4 and all(m > 0 for m in ms)
+    assert a.get_compute_num() == 33
+    a.step(20)
+    b.step(20)
+    assert same_bits(a.read_f(), b.read_f())
+    assert same_bits(a.moments(True)[0], b.moments(True)[0])
