@@ -139,7 +139,7 @@ int lbm_sync(lbm_grid *g);
  * state, restores that state exactly, selects the fastest, and returns it in
  * *best and the per-step time of every variant in ms[LBM_NUM_VARIANTS] (either
  * pointer may be NULL). */
-#define LBM_NUM_VARIANTS 4
+#define LBM_NUM_VARIANTS 8
 int lbm_set_variant(lbm_grid *g, int variant);
 int lbm_get_variant(const lbm_grid *g, int *variant);
 int lbm_autotune(lbm_grid *g, uint32_t steps, int *best, float *ms);

@@ -18,7 +18,7 @@ LBM_ENOMEM = -3
 LBM_ENONFINITE = -4
 LBM_ENODEV = -5
 
-NUM_VARIANTS = 4
+NUM_VARIANTS = 8
 
 PRECISION_F32 = 0
 PRECISION_F16S = 1
