@@ -106,6 +106,35 @@ __global__ void k_build_flags(const uint8_t *__restrict__ mask, uint8_t *__restr
     flags[own] = b;
 }
 
+// The grid is cut into 64-cell segments of a row.  A segment is "clear" when
+// every cell in it is streamed with no bounce-back and lies before the outlet
+// column: the interior kernel then pulls all 64 cells with no per-cell test.
+// Every other segment that holds cells goes on a list that the boundary kernel
+// works through, one wave per segment.  Rows of the segment plane hold
+// pitch / 64 + 8 entries; the extra ones stay 0.
+constexpr int kSegSlack = 8;
+
+__global__ void k_build_segments(const uint8_t *__restrict__ flags, uint8_t *__restrict__ segs,
+                                 uint32_t *__restrict__ list, uint32_t *__restrict__ count, Geom g) {
+    const int srow = g.pitch / 64 + kSegSlack;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (s >= srow) return;
+    uint8_t clear = 0;
+    if (s < g.pitch / 64 && 64 * s + 63 < g.X - 1) {
+        const uint4 *p = (const uint4 *)(flags + (size_t)y * g.pitch + 64 * s);
+        uint32_t acc = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint4 q = p[i];
+            acc |= q.x | q.y | q.z | q.w;
+        }
+        clear = acc == 0;
+    }
+    segs[(size_t)y * srow + s] = clear;
+    if (!clear && 64 * s < g.X) list[atomicAdd(count, 1u)] = (uint32_t)((size_t)y * srow + s);
+}
+
 enum : int {
     OP_COLLIDE = 1,   // collide the gathered state and store all nine planes to dst
     OP_STORE_S = 2,   // store the gathered (streamed) state to dst
@@ -182,7 +211,7 @@ __device__ __forceinline__ bool gather_cell(const typename St::T *src, const typ
                 upstream<MODE>(k, x, y, g.X, ux, uy);
                 const bool bounce = (fl >> bit(k)) & 1u;
                 const size_t idx = bounce ? (size_t)opp(k) * ks + sown : (size_t)k * ks + (size_t)uy * g.ys + ux;
-                f[k] = St::decode(src[idx], bounce ? opp(k) : k);
+                f[k] = St::load(src[idx]);
             }
         } else {
             // enclosed streamed cell: every population bounces back; otherwise
@@ -191,28 +220,92 @@ __device__ __forceinline__ bool gather_cell(const typename St::T *src, const typ
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 if (k == C) continue;
-                f[k] = streamed ? St::decode(src[(size_t)opp(k) * ks + sown], opp(k))
-                                : St::decode(dst[(size_t)k * ks + sown], k);
+                f[k] = streamed ? St::load(src[(size_t)opp(k) * ks + sown]) : St::load(dst[(size_t)k * ks + sown]);
             }
         }
-        f[C] = St::decode(src[(size_t)C * ks + sown], C);
+        f[C] = St::load(src[(size_t)C * ks + sown]);
     } else {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) f[k] = St::decode(src[(size_t)k * ks + sown], k);
+        for (int k = 0; k < 9; ++k) f[k] = St::load(src[(size_t)k * ks + sown]);
     }
     return streamed;
 }
 
+// Per-cell epilogue shared by both kernels: collide and store, store the
+// streamed state, write moments, accumulate the FP64 sums.
+template <class St, int OP>
+__device__ __forceinline__ void finish_cell(float f[9], bool streamed, const typename St::T *src,
+                                            typename St::T *dst, const uint8_t *__restrict__ mask, const Geom &g,
+                                            size_t own, size_t sown, float *__restrict__ mom, double &s0, double &s1,
+                                            double &s2) {
+    const long long ps = g.pstride, ks = g.ks;
+    Moments m;
+    if (OP & OP_STORE_S) {
+        if (streamed) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                if (k != C) dst[(size_t)k * ks + sown] = St::store(f[k]);
+        }
+        dst[(size_t)C * ks + sown] = src[(size_t)C * ks + sown];
+    }
+    if (OP & OP_COLLIDE) {
+        m = St::collide(f, g.omega);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) dst[(size_t)k * ks + sown] = St::store(f[k]);
+    } else {
+        m = St::moments(f, !(OP & OP_NOREST));
+    }
+    if (OP & OP_MOMENTS) {
+        mom[own] = m.jx;
+        mom[ps + own] = m.jy;
+        mom[2 * ps + own] = m.rho;
+    }
+    if ((OP & OP_REDUCE) && mask[own] != 1) {
+        s0 += (double)m.rho;
+        s1 += (double)m.jx;
+        s2 += (double)m.jy;
+    }
+}
+
+// FP64 block sums -> partials[slot * 3 + c]
+__device__ __forceinline__ void block_sums(double s0, double s1, double s2, double *__restrict__ partials,
+                                           size_t slot) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off, 64);
+        s1 += __shfl_xor(s1, off, 64);
+        s2 += __shfl_xor(s2, off, 64);
+    }
+    __shared__ double red[kBlock / 64][3];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[wv][0] = s0;
+        red[wv][1] = s1;
+        red[wv][2] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
+        partials[slot * 3 + threadIdx.x] = t;
+    }
+}
+
+// Interior kernel.  P-form: only cells of clear segments, each thread a group of
+// CPT consecutive cells moved as one vector per population (the nine pulls of a
+// group are contiguous in their upstream rows, at most one element off
+// alignment).  S-form (no pull): every cell, CPT = 1.
 template <class St, int MODE, bool PFORM, int OP, int CPT, bool NT>
 __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, typename St::T *dst,
-                                                 const uint8_t *__restrict__ flags,
-                                                 const uint8_t *__restrict__ mask, Geom g,
+                                                 const uint8_t *__restrict__ mask,
+                                                 const uint8_t *__restrict__ segs, Geom g,
                                                  float *__restrict__ mom, double *__restrict__ partials,
                                                  int blocks_per_row, int nblocks, int remap) {
     using T = typename St::T;
     using V = typename Vec<T, CPT>::a;
     using U = typename Vec<T, CPT>::u;
     using VF = typename Vec<float, CPT>::a;
+    static_assert(PFORM || CPT == 1, "S-form passes run one cell per thread");
     const int L = remap ? xcd_remap(blockIdx.x, nblocks) : (int)blockIdx.x;
     const int y = L / blocks_per_row;
     const int x0 = ((L - y * blocks_per_row) * kBlock + threadIdx.x) * CPT;
@@ -222,27 +315,25 @@ __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, type
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 
     if (x0 < g.X) {
-        // fast path: the whole group lies before the outlet column and (P-form)
-        // every cell is streamed with no bounce-back
-        bool fast = x0 + CPT < g.X;
-        if (PFORM && fast) {
-            if constexpr (CPT == 8) fast = *(const uint64_t *)(flags + own0) == 0;
-            else if constexpr (CPT == 4) fast = *(const uint32_t *)(flags + own0) == 0;
-            else if constexpr (CPT == 2) fast = *(const uint16_t *)(flags + own0) == 0;
-            else fast = flags[own0] == 0;
-        }
-        if (fast) {
-            V v[9];
+        bool go = true;
+        V v[9];
+        if (PFORM) {
+            // the nine loads are issued before the segment test (row and column
+            // clamped into the set, so always in bounds) to overlap its latency
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                if (PFORM) {
-                    const size_t idx = (size_t)(y - cy(k)) * g.ys + (x0 - cx(k));
-                    if (cx(k) == 0) v[k] = vload<V, NT>((const V *)(src + (size_t)k * ks + idx));
-                    else v[k] = (V)vload<U, NT>((const U *)(src + (size_t)k * ks + idx));
-                } else {
-                    v[k] = vload<V, NT>((const V *)(src + (size_t)k * ks + sown0));
-                }
+                const int ry = min(max(y - cy(k), 0), g.Y - 1);
+                const int rx = min(max(x0 - cx(k), 0), g.pitch - CPT);
+                const size_t idx = (size_t)k * ks + (size_t)ry * g.ys + rx;
+                if (cx(k) == 0) v[k] = vload<V, NT>((const V *)(src + idx));
+                else v[k] = (V)vload<U, NT>((const U *)(src + idx));
             }
+            go = segs[(size_t)y * (g.pitch / 64 + kSegSlack) + x0 / 64] == 1;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) v[k] = vload<V, NT>((const V *)(src + (size_t)k * ks + sown0));
+        }
+        if (go) {
             V o[9];
             VF mjx, mjy, mrho;
             uint64_t fluid_bits = ~0ull;
@@ -255,14 +346,14 @@ __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, type
             for (int j = 0; j < CPT; ++j) {
                 float f[9];
 #pragma unroll
-                for (int k = 0; k < 9; ++k) f[k] = St::decode(get_lane<CPT>(v[k], j), k);
+                for (int k = 0; k < 9; ++k) f[k] = St::load(get_lane<CPT>(v[k], j));
                 Moments m;
                 if (OP & OP_COLLIDE) {
-                    m = collide_ref(f, g.omega);
+                    m = St::collide(f, g.omega);
 #pragma unroll
-                    for (int k = 0; k < 9; ++k) set_lane<CPT>(o[k], j, St::encode(f[k], k));
+                    for (int k = 0; k < 9; ++k) set_lane<CPT>(o[k], j, St::store(f[k]));
                 } else {
-                    m = moments_ref(f, !(OP & OP_NOREST));
+                    m = St::moments(f, !(OP & OP_NOREST));
                 }
                 set_lane<CPT>(mjx, j, m.jx);
                 set_lane<CPT>(mjy, j, m.jy);
@@ -286,65 +377,40 @@ __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, type
                 *(VF *)(mom + ps + own0) = mjy;
                 *(VF *)(mom + 2 * ps + own0) = mrho;
             }
-        } else {
-            // general path, one cell at a time
-            for (int j = 0; j < CPT; ++j) {
-                const int x = x0 + j;
-                if (x >= g.X) break;
-                const size_t own = own0 + j, sown = sown0 + j;
-                float f[9];
-                const bool streamed = gather_cell<St, MODE, PFORM>(src, dst, flags, mask, g, x, y, own, sown, f);
-                Moments m;
-                if (OP & OP_STORE_S) {
-                    if (streamed) {
-#pragma unroll
-                        for (int k = 0; k < 9; ++k)
-                            if (k != C) dst[(size_t)k * ks + sown] = St::encode(f[k], k);
-                    }
-                    dst[(size_t)C * ks + sown] = src[(size_t)C * ks + sown];
-                }
-                if (OP & OP_COLLIDE) {
-                    m = collide_ref(f, g.omega);
-#pragma unroll
-                    for (int k = 0; k < 9; ++k) dst[(size_t)k * ks + sown] = St::encode(f[k], k);
-                } else {
-                    m = moments_ref(f, !(OP & OP_NOREST));
-                }
-                if (OP & OP_MOMENTS) {
-                    mom[own] = m.jx;
-                    mom[ps + own] = m.jy;
-                    mom[2 * ps + own] = m.rho;
-                }
-                if ((OP & OP_REDUCE) && mask[own] != 1) {
-                    s0 += (double)m.rho;
-                    s1 += (double)m.jx;
-                    s2 += (double)m.jy;
-                }
-            }
         }
     }
+    if (OP & OP_REDUCE) block_sums(s0, s1, s2, partials, blockIdx.x);
+}
 
-    if (OP & OP_REDUCE) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            s0 += __shfl_xor(s0, off, 64);
-            s1 += __shfl_xor(s1, off, 64);
-            s2 += __shfl_xor(s2, off, 64);
-        }
-        __shared__ double red[kBlock / 64][3];
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        if (lane == 0) {
-            red[wv][0] = s0;
-            red[wv][1] = s1;
-            red[wv][2] = s2;
-        }
-        __syncthreads();
-        if (threadIdx.x < 3) {
-            double t = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
-            partials[(size_t)blockIdx.x * 3 + threadIdx.x] = t;
-        }
+// Boundary kernel (P-form only): the listed segments, one wave per segment and
+// one cell per lane, with the general pull (bounce-back byte, outlet, cells
+// that are not streamed).  A fixed grid strides over the list, whose length is
+// read on the device.
+constexpr int kSlowBlocks = 1024;
+
+template <class St, int MODE, int OP>
+__global__ __launch_bounds__(kBlock) void k_boundary(const typename St::T *src, typename St::T *dst,
+                                                     const uint8_t *__restrict__ flags,
+                                                     const uint8_t *__restrict__ mask,
+                                                     const uint32_t *__restrict__ list,
+                                                     const uint32_t *__restrict__ count, Geom g,
+                                                     float *__restrict__ mom, double *__restrict__ partials) {
+    const uint32_t n = *count;
+    const int srow = g.pitch / 64 + kSegSlack;
+    const int lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (uint32_t e = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); e < n; e += waves) {
+        const uint32_t seg = list[e];
+        const int y = (int)(seg / (uint32_t)srow);
+        const int x = (int)(seg % (uint32_t)srow) * 64 + lane;
+        if (x >= g.X) continue;
+        const size_t own = (size_t)y * g.pitch + x, sown = (size_t)y * g.ys + x;
+        float f[9];
+        const bool streamed = gather_cell<St, MODE, true>(src, dst, flags, mask, g, x, y, own, sown, f);
+        finish_cell<St, OP>(f, streamed, src, dst, mask, g, own, sown, mom, s0, s1, s2);
     }
+    if (OP & OP_REDUCE) block_sums(s0, s1, s2, partials, blockIdx.x);
 }
 
 // one block sums the per-block partials in a fixed order (deterministic)
@@ -378,7 +444,7 @@ __global__ void k_fill(typename St::T *buf, Geom g, float4 v0123, float4 v4567, 
     if (i >= (size_t)g.ys * g.Y) return;
     const float v[9] = {v0123.x, v0123.y, v0123.z, v0123.w, v4567.x, v4567.y, v4567.z, v4567.w, v8};
     const int k = (int)((i / (size_t)g.pitch) % 9);
-    buf[i] = St::encode(v[k], k);
+    buf[i] = St::from_f(v[k], k);
 }
 
 // population k of a set (plane points at element (k, 0, 0)) <-> a dense f32 plane
@@ -387,7 +453,7 @@ __global__ void k_encode_plane(const float *__restrict__ dense, typename St::T *
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= g.X) return;
-    plane[(size_t)y * g.ys + x] = St::encode(dense[(size_t)y * g.X + x], k);
+    plane[(size_t)y * g.ys + x] = St::from_f(dense[(size_t)y * g.X + x], k);
 }
 
 template <class St>
@@ -395,12 +461,12 @@ __global__ void k_decode_plane(const typename St::T *__restrict__ plane, float *
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= g.X) return;
-    dense[(size_t)y * g.X + x] = St::decode(plane[(size_t)y * g.ys + x], k);
+    dense[(size_t)y * g.X + x] = St::to_f(plane[(size_t)y * g.ys + x], k);
 }
 
 template <class St>
 __global__ void k_set_one(typename St::T *p, float v, int k) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = St::encode(v, k);
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = St::from_f(v, k);
 }
 
 __global__ void k_paint(const uint32_t *__restrict__ idx, const uint8_t *__restrict__ val, size_t n,
@@ -515,7 +581,8 @@ struct lbm_grid {
     void *f[2] = {nullptr, nullptr};
     int cur = 0;
     bool pform = false;
-    uint8_t *mask = nullptr, *flags = nullptr;
+    uint8_t *mask = nullptr, *flags = nullptr, *segs = nullptr;
+    uint32_t *slist = nullptr, *scount = nullptr;  // boundary segment list and its length
     float *lag = nullptr;   // jx, jy, rho planes: the renderer's density bind group
     float *now = nullptr;   // moments of the current state (scratch)
     float *out = nullptr;   // summary output plane (persistent)
@@ -539,11 +606,12 @@ struct lbm_grid {
 
     void release() {
         (void)hipSetDevice(dev);
-        for (void *p : {f[0], f[1], (void *)mask, (void *)flags, (void *)lag, (void *)now, (void *)out,
-                        (void *)dense, (void *)partials, (void *)sums})
+        for (void *p : {f[0], f[1], (void *)mask, (void *)flags, (void *)segs, (void *)slist, (void *)scount,
+                        (void *)lag, (void *)now, (void *)out, (void *)dense, (void *)partials, (void *)sums})
             if (p) (void)hipFree(p);
         f[0] = f[1] = nullptr;
-        mask = flags = nullptr;
+        mask = flags = segs = nullptr;
+        slist = scount = nullptr;
         lag = now = out = dense = nullptr;
         partials = sums = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
@@ -551,15 +619,27 @@ struct lbm_grid {
     }
 
     // ---- kernel launchers --------------------------------------------------
+    // One pass: the interior kernel, then (P-form) the boundary kernel.  They
+    // write disjoint cells and read only src and their own cells of dst.  FP64
+    // partial sums go to part[0, nblocks) and part[nblocks, nblocks + kSlowBlocks).
     template <class St, int MODE, bool PFORM, int OP, int CPT, bool NT>
     void go(const void *src, void *dst, float *mom, double *part, int remap) {
         using T = typename St::T;
         const Shape sh = shape(CPT);
         hipLaunchKernelGGL((k_cell<St, MODE, PFORM, OP, CPT, NT>), dim3(sh.nblocks), dim3(kBlock), 0, stream,
-                           (const T *)src, (T *)dst, flags, mask, geom, mom, part, sh.blocks_per_row, sh.nblocks,
+                           (const T *)src, (T *)dst, mask, segs, geom, mom, part, sh.blocks_per_row, sh.nblocks,
                            remap);
         HIPCHK(hipGetLastError());
+        last_partials = (size_t)sh.nblocks;
+        if constexpr (PFORM) {
+            hipLaunchKernelGGL((k_boundary<St, MODE, OP>), dim3(kSlowBlocks), dim3(kBlock), 0, stream,
+                               (const T *)src, (T *)dst, flags, mask, slist, scount, geom, mom,
+                               part ? part + 3 * (size_t)sh.nblocks : nullptr);
+            HIPCHK(hipGetLastError());
+            last_partials += kSlowBlocks;
+        }
     }
+    size_t last_partials = 0;
 
     // cells per thread of the non-step passes
     static constexpr int default_cpt(int precision) { return precision == LBM_PRECISION_F32 ? 1 : 2; }
@@ -590,7 +670,7 @@ struct lbm_grid {
                 else return go<St, MODE, PFORM, OP, 8, true>(src, dst, mom, part, 0);
             }
         }
-        go<St, MODE, PFORM, OP, (f32 ? 1 : 2), true>(src, dst, mom, part, 0);
+        go<St, MODE, PFORM, OP, (PFORM && !f32 ? 2 : 1), true>(src, dst, mom, part, 0);
     }
 
     template <bool PFORM, int OP>
@@ -616,6 +696,11 @@ struct lbm_grid {
             hipLaunchKernelGGL(k_build_flags<LBM_BOUNDARY_ZERO_GRADIENT>, grid, dim3(256), 0, stream, mask, flags,
                                geom);
         HIPCHK(hipGetLastError());
+        const int srow = geom.pitch / 64 + kSegSlack;
+        HIPCHK(hipMemsetAsync(scount, 0, sizeof(uint32_t), stream));
+        hipLaunchKernelGGL(k_build_segments, dim3((srow + 255) / 256, Y), dim3(256), 0, stream, flags, segs, slist,
+                           scount, geom);
+        HIPCHK(hipGetLastError());
     }
 
     void fill_both(const float v[9]) {
@@ -640,7 +725,7 @@ struct lbm_grid {
     }
     void ensure_partials() {
         if (!partials) {
-            npartials = (size_t)shape(default_cpt(precision)).nblocks;
+            npartials = (size_t)shape(1).nblocks + kSlowBlocks;  // the largest pass
             partials = dalloc<double>(3 * npartials);
             sums = dalloc<double>(3);
         }
@@ -825,6 +910,9 @@ int lbm_create(const lbm_config *cfg, lbm_grid **out) {
         g->f[1] = dalloc<char>(g->set_bytes());
         g->mask = dalloc<uint8_t>(pstride);
         g->flags = dalloc<uint8_t>(pstride);
+        g->segs = dalloc<uint8_t>((size_t)(pitch / 64 + kSegSlack) * cfg->ny);
+        g->slist = dalloc<uint32_t>((size_t)(pitch / 64) * cfg->ny);
+        g->scount = dalloc<uint32_t>(1);
         g->lag = dalloc<float>(3 * (size_t)pstride);
         g->out = dalloc<float>(pstride);
         HIPCHK(hipMemsetAsync(g->lag, 0, sizeof(float) * 3 * pstride, g->stream));
@@ -1162,7 +1250,8 @@ int lbm_conservation(lbm_grid *g, double out[3]) {
     int rc = guarded(g, [&] {
         check(out != nullptr, "null output");
         g->moments_now(true);
-        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, g->stream, g->partials, g->npartials, g->sums);
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, g->stream, g->partials, g->last_partials,
+                           g->sums);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(out, g->sums, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
         HIPCHK(hipStreamSynchronize(g->stream));
