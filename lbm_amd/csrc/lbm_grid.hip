@@ -44,341 +44,25 @@
 #include <unordered_map>
 #include <vector>
 
-#include "../../include/lbm_hip.h"
-#include "d2q9.h"
-
-using namespace d2q9;
+#include "lbm_common.h"
 
 namespace {
 
-// ---------------------------------------------------------------------------
-// device side
-// ---------------------------------------------------------------------------
-
-struct Geom {
-    int X, Y;
-    int pitch;          // elements per row of every plane
-    long long pstride;  // elements per auxiliary plane (mask, flags, moments)
-    long long ys, ks;   // state strides: element (k, x, y) of a set is at y * ys + k * ks + x
-    float omega;
-};
-
-// per-cell bounce-back byte: bit(k) set if population k's upstream is solid or
-// outside the grid; 0xFF marks a cell the fast path does not pull (not streamed,
-// or streamed with all eight upstream cells solid)
-constexpr uint8_t kSlowCell = 0xFF;
-
-// upstream cell of population k at (x, y); mode 0 wraps the flattened index at
-// the outlet column, mode 1 clamps it (zero-gradient outflow)
-template <int MODE>
-__host__ __device__ __forceinline__ void upstream(int k, int x, int y, int X, int &ux, int &uy) {
-    ux = x - cx(k);
-    uy = y - cy(k);
-    if (ux >= X) {
-        if (MODE == LBM_BOUNDARY_REFERENCE) {
-            ux = 0;
-            uy += 1;
-        } else {
-            ux = X - 1;
-        }
-    }
-}
-
-template <int MODE>
-__global__ void k_build_flags(const uint8_t *__restrict__ mask, uint8_t *__restrict__ flags, Geom g) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= g.X) return;
-    const size_t own = (size_t)y * g.pitch + x;
-    const bool streamed = mask[own] != 1 && x != 0 && y < g.Y - 1;
-    uint8_t b = kSlowCell;
-    if (streamed) {
-        b = 0;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            if (k == C) continue;
-            int ux, uy;
-            upstream<MODE>(k, x, y, g.X, ux, uy);
-            const bool solid = uy < 0 || uy >= g.Y || ux < 0 || mask[(size_t)uy * g.pitch + ux] == 1;
-            if (solid) b |= (uint8_t)(1u << bit(k));
-        }
-    }
-    flags[own] = b;
-}
-
-// The grid is cut into 64-cell segments of a row.  A segment is "clear" when
-// every cell in it is streamed with no bounce-back and lies before the outlet
-// column: the interior kernel then pulls all 64 cells with no per-cell test.
-// Every other segment that holds cells goes on a list that the boundary kernel
-// works through, one wave per segment.  Rows of the segment plane hold
-// pitch / 64 + 8 entries; the extra ones stay 0.
-constexpr int kSegSlack = 8;
-
-__global__ void k_build_segments(const uint8_t *__restrict__ flags, uint8_t *__restrict__ segs,
-                                 uint32_t *__restrict__ list, uint32_t *__restrict__ count, Geom g) {
-    const int srow = g.pitch / 64 + kSegSlack;
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (s >= srow) return;
-    uint8_t clear = 0;
-    if (s < g.pitch / 64 && 64 * s + 63 < g.X - 1) {
-        const uint4 *p = (const uint4 *)(flags + (size_t)y * g.pitch + 64 * s);
-        uint32_t acc = 0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const uint4 q = p[i];
-            acc |= q.x | q.y | q.z | q.w;
-        }
-        clear = acc == 0;
-    }
-    segs[(size_t)y * srow + s] = clear;
-    if (!clear && 64 * s < g.X) list[atomicAdd(count, 1u)] = (uint32_t)((size_t)y * srow + s);
-}
-
-enum : int {
-    OP_COLLIDE = 1,   // collide the gathered state and store all nine planes to dst
-    OP_STORE_S = 2,   // store the gathered (streamed) state to dst
-    OP_MOMENTS = 4,   // write jx, jy, rho of the gathered state
-    OP_REDUCE = 8,    // FP64 partial sums of rho, jx, jy over fluid cells
-    OP_NOREST = 16,   // moments without the rest population
-};
-
-constexpr int kBlock = 256;
-
-// CPT consecutive cells of a row per thread, moved as one vector: aligned for the
-// cell's own planes, element-aligned for the planes pulled from x - 1 / x + 1
-template <class T, int N>
-struct Vec {
-    typedef T a __attribute__((ext_vector_type(N)));
-    typedef T u __attribute__((ext_vector_type(N), aligned(sizeof(T))));
-};
-template <class T>
-struct Vec<T, 1> {
-    typedef T a;
-    typedef T u;
-};
-
-template <int N, class V>
-__device__ __forceinline__ auto get_lane(const V &v, int j) {
-    if constexpr (N == 1) return v;
-    else return v[j];
-}
-
-template <int N, class V, class E>
-__device__ __forceinline__ void set_lane(V &v, int j, E x) {
-    if constexpr (N == 1) v = x;
-    else v[j] = x;
-}
-
-template <class V, bool NT>
-__device__ __forceinline__ V vload(const V *p) {
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
-}
-
-template <class V, bool NT>
-__device__ __forceinline__ void vstore(V v, V *p) {
-    if constexpr (NT) __builtin_nontemporal_store(v, p);
-    else *p = v;
-}
-
-// Workgroup ids are handed to the eight XCDs in turn; remap them so that each
-// XCD walks one contiguous band of rows.  Neighbouring row segments, which
-// share the cache lines of their x - 1 / x + 1 pulls, then meet in one L2.
-__device__ __forceinline__ int xcd_remap(int b, int n) {
-    const int q = n >> 3;
-    if (b >= (q << 3)) return b;
-    return (b & 7) * q + (b >> 3);
-}
-
-// The state S of one cell before collision.  P-form: pull from src with the
-// bounce-back byte, or, for cells the stream does not write, keep dst's own
-// values; the rest population is src's.  S-form: src's own values.
-template <class St, int MODE, bool PFORM>
-__device__ __forceinline__ bool gather_cell(const typename St::T *src, const typename St::T *dst,
-                                            const uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask,
-                                            const Geom &g, int x, int y, size_t own, size_t sown, float f[9]) {
-    const long long ks = g.ks;
-    bool streamed = false;
-    if (PFORM) {
-        const uint8_t fl = flags[own];
-        if (fl != kSlowCell) {
-            streamed = true;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                if (k == C) continue;
-                int ux, uy;
-                upstream<MODE>(k, x, y, g.X, ux, uy);
-                const bool bounce = (fl >> bit(k)) & 1u;
-                const size_t idx = bounce ? (size_t)opp(k) * ks + sown : (size_t)k * ks + (size_t)uy * g.ys + ux;
-                f[k] = St::load(src[idx]);
-            }
-        } else {
-            // enclosed streamed cell: every population bounces back; otherwise
-            // the cell is not streamed and keeps the other set's values
-            streamed = mask[own] != 1 && x != 0 && y < g.Y - 1;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                if (k == C) continue;
-                f[k] = streamed ? St::load(src[(size_t)opp(k) * ks + sown]) : St::load(dst[(size_t)k * ks + sown]);
-            }
-        }
-        f[C] = St::load(src[(size_t)C * ks + sown]);
-    } else {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) f[k] = St::load(src[(size_t)k * ks + sown]);
-    }
-    return streamed;
-}
-
-// Per-cell epilogue shared by both kernels: collide and store, store the
-// streamed state, write moments, accumulate the FP64 sums.
-template <class St, int OP>
-__device__ __forceinline__ void finish_cell(float f[9], bool streamed, const typename St::T *src,
-                                            typename St::T *dst, const uint8_t *__restrict__ mask, const Geom &g,
-                                            size_t own, size_t sown, float *__restrict__ mom, double &s0, double &s1,
-                                            double &s2) {
-    const long long ps = g.pstride, ks = g.ks;
-    Moments m;
-    if (OP & OP_STORE_S) {
-        if (streamed) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k)
-                if (k != C) dst[(size_t)k * ks + sown] = St::store(f[k]);
-        }
-        dst[(size_t)C * ks + sown] = src[(size_t)C * ks + sown];
-    }
-    if (OP & OP_COLLIDE) {
-        m = St::collide(f, g.omega);
-#pragma unroll
-        for (int k = 0; k < 9; ++k) dst[(size_t)k * ks + sown] = St::store(f[k]);
-    } else {
-        m = St::moments(f, !(OP & OP_NOREST));
-    }
-    if (OP & OP_MOMENTS) {
-        mom[own] = m.jx;
-        mom[ps + own] = m.jy;
-        mom[2 * ps + own] = m.rho;
-    }
-    if ((OP & OP_REDUCE) && mask[own] != 1) {
-        s0 += (double)m.rho;
-        s1 += (double)m.jx;
-        s2 += (double)m.jy;
-    }
-}
-
-// FP64 block sums -> partials[slot * 3 + c]
-__device__ __forceinline__ void block_sums(double s0, double s1, double s2, double *__restrict__ partials,
-                                           size_t slot) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        s0 += __shfl_xor(s0, off, 64);
-        s1 += __shfl_xor(s1, off, 64);
-        s2 += __shfl_xor(s2, off, 64);
-    }
-    __shared__ double red[kBlock / 64][3];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) {
-        red[wv][0] = s0;
-        red[wv][1] = s1;
-        red[wv][2] = s2;
-    }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        double t = 0.0;
-        for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
-        partials[slot * 3 + threadIdx.x] = t;
-    }
-}
-
-// Interior kernel.  P-form: only cells of clear segments, each thread a group of
-// CPT consecutive cells moved as one vector per population (the nine pulls of a
-// group are contiguous in their upstream rows, at most one element off
-// alignment).  S-form (no pull): every cell, CPT = 1.
+// Interior kernel of one grid.  P-form: cells of clear segments only; S-form:
+// every cell (CPT = 1).
 template <class St, int MODE, bool PFORM, int OP, int CPT, bool NT>
 __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, typename St::T *dst,
                                                  const uint8_t *__restrict__ mask,
                                                  const uint8_t *__restrict__ segs, Geom g,
                                                  float *__restrict__ mom, double *__restrict__ partials,
                                                  int blocks_per_row, int nblocks, int remap) {
-    using T = typename St::T;
-    using V = typename Vec<T, CPT>::a;
-    using U = typename Vec<T, CPT>::u;
-    using VF = typename Vec<float, CPT>::a;
     static_assert(PFORM || CPT == 1, "S-form passes run one cell per thread");
     const int L = remap ? xcd_remap(blockIdx.x, nblocks) : (int)blockIdx.x;
     const int y = L / blocks_per_row;
     const int x0 = ((L - y * blocks_per_row) * kBlock + threadIdx.x) * CPT;
-    const long long ps = g.pstride, ks = g.ks;
-    const size_t own0 = (size_t)y * g.pitch + x0;  // auxiliary planes
-    const size_t sown0 = (size_t)y * g.ys + x0;    // state
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-
-    if (x0 < g.X) {
-        bool go = true;
-        V v[9];
-        if (PFORM) {
-            // the nine loads are issued before the segment test (row and column
-            // clamped into the set, so always in bounds) to overlap its latency
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                const int ry = min(max(y - cy(k), 0), g.Y - 1);
-                const int rx = min(max(x0 - cx(k), 0), g.pitch - CPT);
-                const size_t idx = (size_t)k * ks + (size_t)ry * g.ys + rx;
-                if (cx(k) == 0) v[k] = vload<V, NT>((const V *)(src + idx));
-                else v[k] = (V)vload<U, NT>((const U *)(src + idx));
-            }
-            go = segs[(size_t)y * (g.pitch / 64 + kSegSlack) + x0 / 64] == 1;
-        } else {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) v[k] = vload<V, NT>((const V *)(src + (size_t)k * ks + sown0));
-        }
-        if (go) {
-            V o[9];
-            VF mjx, mjy, mrho;
-            uint64_t fluid_bits = ~0ull;
-            if ((OP & OP_REDUCE) && !PFORM) {
-                fluid_bits = 0;
-#pragma unroll
-                for (int j = 0; j < CPT; ++j) fluid_bits |= (uint64_t)(mask[own0 + j] != 1) << j;
-            }
-#pragma unroll
-            for (int j = 0; j < CPT; ++j) {
-                float f[9];
-#pragma unroll
-                for (int k = 0; k < 9; ++k) f[k] = St::load(get_lane<CPT>(v[k], j));
-                Moments m;
-                if (OP & OP_COLLIDE) {
-                    m = St::collide(f, g.omega);
-#pragma unroll
-                    for (int k = 0; k < 9; ++k) set_lane<CPT>(o[k], j, St::store(f[k]));
-                } else {
-                    m = St::moments(f, !(OP & OP_NOREST));
-                }
-                set_lane<CPT>(mjx, j, m.jx);
-                set_lane<CPT>(mjy, j, m.jy);
-                set_lane<CPT>(mrho, j, m.rho);
-                if ((OP & OP_REDUCE) && ((fluid_bits >> j) & 1)) {
-                    s0 += (double)m.rho;
-                    s1 += (double)m.jx;
-                    s2 += (double)m.jy;
-                }
-            }
-            if (OP & OP_COLLIDE) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) vstore<V, NT>(o[k], (V *)(dst + (size_t)k * ks + sown0));
-            }
-            if (OP & OP_STORE_S) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) vstore<V, NT>(v[k], (V *)(dst + (size_t)k * ks + sown0));
-            }
-            if (OP & OP_MOMENTS) {
-                *(VF *)(mom + own0) = mjx;
-                *(VF *)(mom + ps + own0) = mjy;
-                *(VF *)(mom + 2 * ps + own0) = mrho;
-            }
-        }
-    }
+    if (x0 < g.X)
+        interior_tile<St, MODE, PFORM, OP, CPT, NT, false>(src, dst, mask, segs, g, y, x0, mom, s0, s1, s2);
     if (OP & OP_REDUCE) block_sums(s0, s1, s2, partials, blockIdx.x);
 }
 
@@ -413,31 +97,6 @@ __global__ __launch_bounds__(kBlock) void k_boundary(const typename St::T *src, 
     if (OP & OP_REDUCE) block_sums(s0, s1, s2, partials, blockIdx.x);
 }
 
-// one block sums the per-block partials in a fixed order (deterministic)
-__global__ __launch_bounds__(1024) void k_reduce_final(const double *__restrict__ partials, size_t n,
-                                                       double *__restrict__ out) {
-    __shared__ double red[1024 / 64][3];
-    double s[3] = {0.0, 0.0, 0.0};
-    for (size_t i = threadIdx.x; i < n; i += blockDim.x) {
-        s[0] += partials[i * 3 + 0];
-        s[1] += partials[i * 3 + 1];
-        s[2] += partials[i * 3 + 2];
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-        for (int off = 32; off > 0; off >>= 1) s[c] += __shfl_xor(s[c], off, 64);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0)
-        for (int c = 0; c < 3; ++c) red[wv][c] = s[c];
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        double t = 0.0;
-        for (int w = 0; w < (int)(blockDim.x / 64); ++w) t += red[w][threadIdx.x];
-        out[threadIdx.x] = t;
-    }
-}
-
-// every element of one set (9 * pitch * ny): row slot i / pitch = 9 y + k
 template <class St>
 __global__ void k_fill(typename St::T *buf, Geom g, float4 v0123, float4 v4567, float v8) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -536,36 +195,6 @@ __global__ void k_fields(const float *__restrict__ mom, float *__restrict__ outp
         curl = 10.0f * (uye - uyw - uxn + uxs) / r;
     }
     outp[3 * dn + d] = curl;
-}
-
-// ---------------------------------------------------------------------------
-// host side
-// ---------------------------------------------------------------------------
-
-thread_local std::string g_create_error;
-
-struct HipError {
-    hipError_t e;
-    const char *what;
-};
-
-#define HIPCHK(expr)                                  \
-    do {                                              \
-        hipError_t e_ = (expr);                       \
-        if (e_ != hipSuccess) throw HipError{e_, #expr}; \
-    } while (0)
-
-struct ArgError {
-    std::string msg;
-};
-
-template <class T>
-T *dalloc(size_t n) {
-    void *p = nullptr;
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc(&p, n * sizeof(T));
-    if (e != hipSuccess) throw HipError{e, "hipMalloc"};
-    return (T *)p;
 }
 
 }  // namespace
