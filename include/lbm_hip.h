@@ -190,6 +190,49 @@ int lbm_state_size(const lbm_grid *g, size_t *bytes);
 int lbm_save_state(lbm_grid *g, void *buf, size_t bytes);
 int lbm_load_state(lbm_grid *g, const void *buf, size_t bytes);
 
+/* ---- batched domains (no reference counterpart: the reference steps one grid) -------- */
+
+/* Many independent grids of uneven size stepped together: each domain follows
+ * the single-grid semantics above (walls on rows 0 and ny-1, equilibrium inlet,
+ * the selected outlet) with its own omega and inflow speed, and equals a single
+ * grid of the same size and mask bit for bit.  All domains share one precision,
+ * one boundary mode and one step count.  A step launches one interior and one
+ * boundary kernel for all domains; lbm_batch_step replays them from a hipGraph
+ * two steps at a time. */
+typedef struct lbm_batch lbm_batch;
+
+typedef struct lbm_domain {
+    uint32_t nx, ny;
+    float omega;  /* 1/tau */
+    float u_in;   /* uniform x velocity of the initial / inflow equilibrium */
+} lbm_domain;
+
+#define LBM_BATCH_NUM_VARIANTS 6
+
+int lbm_batch_create(const lbm_domain *domains, uint32_t n, int32_t precision, int32_t boundary,
+                     int32_t device, lbm_batch **out);
+int lbm_batch_destroy(lbm_batch *b);
+const char *lbm_batch_last_error(const lbm_batch *b);
+/* every domain := feq(1, (u_in, 0)) in both sets; step count := 0 */
+int lbm_batch_reset(lbm_batch *b);
+/* whole-domain mask, one byte per cell, 1 = solid (like lbm_set_mask) */
+int lbm_batch_set_mask(lbm_batch *b, uint32_t domain, const uint8_t *solid);
+int lbm_batch_step(lbm_batch *b, uint32_t n);
+int lbm_batch_step_timed(lbm_batch *b, uint32_t n, float *ms);
+int lbm_batch_sync(lbm_batch *b);
+int lbm_batch_step_count(const lbm_batch *b, uint64_t *n);
+/* total cells, interior tiles and boundary segments of the current work lists */
+int lbm_batch_info(lbm_batch *b, uint64_t *cells, uint32_t *interior_tiles, uint32_t *boundary_segments);
+/* hipGraph replay on (default) or off */
+int lbm_batch_set_graph(lbm_batch *b, int enable);
+/* interior-kernel tile variants (cells per thread, list order); autotune as lbm_autotune */
+int lbm_batch_set_variant(lbm_batch *b, int variant);
+int lbm_batch_get_variant(const lbm_batch *b, int *variant);
+int lbm_batch_autotune(lbm_batch *b, uint32_t steps, int *best, float *ms);
+/* one domain's current state (9 dense nx*ny f32 planes) and renderer (lagged) moments */
+int lbm_batch_read_f(lbm_batch *b, uint32_t domain, float *f9);
+int lbm_batch_read_moments(lbm_batch *b, uint32_t domain, float *jx, float *jy, float *rho);
+
 /* ---- seeded obstacle generator (host only, no GPU) --------------------------------- */
 
 /* SplitMix64 (Steele, Lea and Flood 2014).  Returns the next 64-bit output and
@@ -204,6 +247,17 @@ uint64_t lbm_splitmix64(uint64_t *state);
  * and keep column 0 free (cx - r >= 1). */
 int lbm_gen_circles(uint8_t *solid, uint32_t nx, uint32_t ny, uint64_t seed,
                     uint32_t n_circles, uint32_t r_min, uint32_t r_max);
+
+/* n batch domains from one seed: nx = 128 + 32 u(29), ny = 64 + 32 u(15),
+ * tau = 0.52 + 0.28 d, u_in = 0.02 + 0.08 d, drawn in that order per domain, with
+ * d = (splitmix64() >> 11) * 2^-53 and u as for lbm_gen_circles; omega = 1/tau. */
+int lbm_gen_batch(uint64_t seed, uint32_t n, lbm_domain *out);
+
+/* n_min + u(n_max - n_min + 1) shapes into solid[] (nx, ny >= 16), each a circle
+ * (u(2) == 0: r = 3 + u(min(nx, ny) / 8), cx = r + 1 + u(nx - r - 1), cy = u(ny)) or an
+ * axis-aligned rectangle (w = 2 + u(nx / 8), h = 2 + u(ny / 4), x0 = 1 + u(nx - w - 1),
+ * y0 = u(ny - h)); column 0 stays free. */
+int lbm_gen_shapes(uint8_t *solid, uint32_t nx, uint32_t ny, uint64_t seed, uint32_t n_min, uint32_t n_max);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,7 @@ LBM_ENONFINITE = -4
 LBM_ENODEV = -5
 
 NUM_VARIANTS = 8
+BATCH_NUM_VARIANTS = 6
 
 PRECISION_F32 = 0
 PRECISION_F16S = 1
@@ -42,6 +43,15 @@ class lbm_config(ctypes.Structure):
         ("boundary", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("row_pad", ctypes.c_uint32),
+    ]
+
+
+class lbm_domain(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_uint32),
+        ("ny", ctypes.c_uint32),
+        ("omega", ctypes.c_float),
+        ("u_in", ctypes.c_float),
     ]
 
 
@@ -87,6 +97,24 @@ SIGNATURES = {
     "lbm_state_size": (_i32, [_P, ctypes.POINTER(_sz)]),
     "lbm_save_state": (_i32, [_P, _P, _sz]),
     "lbm_load_state": (_i32, [_P, _P, _sz]),
+    "lbm_batch_create": (_i32, [ctypes.POINTER(lbm_domain), _u32, _i32, _i32, _i32, _PP]),
+    "lbm_batch_destroy": (_i32, [_P]),
+    "lbm_batch_last_error": (ctypes.c_char_p, [_P]),
+    "lbm_batch_reset": (_i32, [_P]),
+    "lbm_batch_set_mask": (_i32, [_P, _u32, _bp]),
+    "lbm_batch_step": (_i32, [_P, _u32]),
+    "lbm_batch_step_timed": (_i32, [_P, _u32, _fp]),
+    "lbm_batch_sync": (_i32, [_P]),
+    "lbm_batch_step_count": (_i32, [_P, _u64p]),
+    "lbm_batch_info": (_i32, [_P, _u64p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
+    "lbm_batch_set_graph": (_i32, [_P, _i32]),
+    "lbm_batch_set_variant": (_i32, [_P, _i32]),
+    "lbm_batch_get_variant": (_i32, [_P, ctypes.POINTER(ctypes.c_int)]),
+    "lbm_batch_autotune": (_i32, [_P, _u32, ctypes.POINTER(ctypes.c_int), _fp]),
+    "lbm_batch_read_f": (_i32, [_P, _u32, _fp]),
+    "lbm_batch_read_moments": (_i32, [_P, _u32, _fp, _fp, _fp]),
+    "lbm_gen_batch": (_i32, [_u64, _u32, ctypes.POINTER(lbm_domain)]),
+    "lbm_gen_shapes": (_i32, [_bp, _u32, _u32, _u64, _u32, _u32]),
     "lbm_splitmix64": (_u64, [_u64p]),
     "lbm_gen_circles": (_i32, [_bp, _u32, _u32, _u64, _u32, _u32, _u32]),
 }
@@ -115,5 +143,12 @@ def load(path: str = LIB_PATH):
 def check(code: int, handle=None):
     if code != LBM_OK:
         msg = load().lbm_last_error(handle)
+        raise LbmError(code, msg.decode() if msg else "")
+    return code
+
+
+def check_batch(code: int, handle=None):
+    if code != LBM_OK:
+        msg = load().lbm_batch_last_error(handle)
         raise LbmError(code, msg.decode() if msg else "")
     return code

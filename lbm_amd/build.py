@@ -22,7 +22,7 @@ SOURCES = ["lbm_grid.hip", "lbm_batch.hip", "unet.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
-    "-std=c++17",
+    "-std=c++20",
     "-fPIC",
     "-shared",
     "-ffp-contract=off",

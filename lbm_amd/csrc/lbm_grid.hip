@@ -97,106 +97,6 @@ __global__ __launch_bounds__(kBlock) void k_boundary(const typename St::T *src, 
     if (OP & OP_REDUCE) block_sums(s0, s1, s2, partials, blockIdx.x);
 }
 
-template <class St>
-__global__ void k_fill(typename St::T *buf, Geom g, float4 v0123, float4 v4567, float v8) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)g.ys * g.Y) return;
-    const float v[9] = {v0123.x, v0123.y, v0123.z, v0123.w, v4567.x, v4567.y, v4567.z, v4567.w, v8};
-    const int k = (int)((i / (size_t)g.pitch) % 9);
-    buf[i] = St::from_f(v[k], k);
-}
-
-// population k of a set (plane points at element (k, 0, 0)) <-> a dense f32 plane
-template <class St>
-__global__ void k_encode_plane(const float *__restrict__ dense, typename St::T *__restrict__ plane, Geom g, int k) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= g.X) return;
-    plane[(size_t)y * g.ys + x] = St::from_f(dense[(size_t)y * g.X + x], k);
-}
-
-template <class St>
-__global__ void k_decode_plane(const typename St::T *__restrict__ plane, float *__restrict__ dense, Geom g, int k) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= g.X) return;
-    dense[(size_t)y * g.X + x] = St::to_f(plane[(size_t)y * g.ys + x], k);
-}
-
-template <class St>
-__global__ void k_set_one(typename St::T *p, float v, int k) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = St::from_f(v, k);
-}
-
-__global__ void k_paint(const uint32_t *__restrict__ idx, const uint8_t *__restrict__ val, size_t n,
-                        uint8_t *__restrict__ mask, int X, int pitch) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t c = idx[i];
-    mask[(size_t)(c / X) * pitch + (c % X)] = val[i];
-}
-
-__global__ void k_walls(uint8_t *__restrict__ mask, Geom g) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= g.X) return;
-    mask[x] = 1;
-    mask[(size_t)(g.Y - 1) * g.pitch + x] = 1;
-}
-
-// summary passes (summary_stats/{curl,ux,uy,rho,speed}.wgsl); out keeps skipped cells
-__global__ void k_summary(const float *__restrict__ mom, float *__restrict__ out, Geom g, int kind) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= g.X) return;
-    const long long ps = g.pstride;
-    const float *jx = mom, *jy = mom + ps, *rho = mom + 2 * ps;
-    const size_t i = (size_t)y * g.pitch + x;
-    float v;
-    switch (kind) {
-    case LBM_SUMMARY_CURL: {
-        if (x == 0 || y >= g.Y - 1) return;
-        // flattened neighbours: x+1 at the last column is column 0 of the next row
-        const int xe = x + 1 < g.X ? x + 1 : 0, ye = x + 1 < g.X ? y : y + 1;
-        const float uye = jy[(size_t)ye * g.pitch + xe];
-        const float uyw = jy[i - 1];
-        const float uxn = y > 0 ? jx[i - g.pitch] : 0.0f;
-        const float uxs = jx[i + g.pitch];
-        v = 10.0f * (uye - uyw - uxn + uxs) / rho[i];
-        break;
-    }
-    case LBM_SUMMARY_UX: v = jx[i]; break;
-    case LBM_SUMMARY_UY: v = jy[i]; break;
-    case LBM_SUMMARY_RHO: v = 4.0f * fminf(fmaxf(0.15f * rho[i], 0.0f), 1.0f) - 0.5f; break;
-    default: v = fminf(fmaxf(5.0f * sqrtf(jx[i] * jx[i] + jy[i] * jy[i]), 0.0f), 1.0f) - 0.5f; break;
-    }
-    out[i] = v;
-}
-
-// rho, u = j / rho, curl into four dense planes
-__global__ void k_fields(const float *__restrict__ mom, float *__restrict__ outp, Geom g) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= g.X) return;
-    const long long ps = g.pstride;
-    const float *jx = mom, *jy = mom + ps, *rho = mom + 2 * ps;
-    const size_t i = (size_t)y * g.pitch + x;
-    const size_t d = (size_t)y * g.X + x, dn = (size_t)g.X * g.Y;
-    const float r = rho[i];
-    outp[d] = r;
-    outp[dn + d] = jx[i] / r;
-    outp[2 * dn + d] = jy[i] / r;
-    float curl = 0.0f;
-    if (x != 0 && y < g.Y - 1) {
-        const int xe = x + 1 < g.X ? x + 1 : 0, ye = x + 1 < g.X ? y : y + 1;
-        const float uye = jy[(size_t)ye * g.pitch + xe];
-        const float uyw = jy[i - 1];
-        const float uxn = y > 0 ? jx[i - g.pitch] : 0.0f;
-        const float uxs = jx[i + g.pitch];
-        curl = 10.0f * (uye - uyw - uxn + uxs) / r;
-    }
-    outp[3 * dn + d] = curl;
-}
-
 }  // namespace
 
 struct lbm_grid {
