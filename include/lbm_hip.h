@@ -233,6 +233,21 @@ int lbm_batch_autotune(lbm_batch *b, uint32_t steps, int *best, float *ms);
 int lbm_batch_read_f(lbm_batch *b, uint32_t domain, float *f9);
 int lbm_batch_read_moments(lbm_batch *b, uint32_t domain, float *jx, float *jy, float *rho);
 
+/* ---- learned surrogate (no reference counterpart) --------------------------------- */
+
+/* Text of the last error of an lbm_unet_* / lbm_conv3x3_test call on this thread. */
+const char *lbm_unet_last_error(void);
+
+/* The surrogate's 3x3 convolution primitives on host arrays, for testing: NHWC
+ * f32 tensors rounded to bf16 on the device, f32 accumulation on the matrix
+ * cores.  w is [9][Cout][Cin] (tap = 3 dy + dx).  mode 0: out (N,H,W,Cout) =
+ * conv(in) + bias (bias may be NULL); mode 1: out (N,H,W,Cin) = input gradient
+ * for the output gradient `in` (N,H,W,Cout); mode 2: out [9][Cout][Cin] = weight
+ * gradient for input `in` and output gradient `g`.  W, Cin, Cout must be
+ * multiples of 32. */
+int lbm_conv3x3_test(int mode, int n, int h, int w, int cin, int cout, const float *in, const float *g,
+                     const float *weights, const float *bias, float *out, int device);
+
 /* ---- seeded obstacle generator (host only, no GPU) --------------------------------- */
 
 /* SplitMix64 (Steele, Lea and Flood 2014).  Returns the next 64-bit output and

@@ -115,6 +115,8 @@ SIGNATURES = {
     "lbm_batch_read_moments": (_i32, [_P, _u32, _fp, _fp, _fp]),
     "lbm_gen_batch": (_i32, [_u64, _u32, ctypes.POINTER(lbm_domain)]),
     "lbm_gen_shapes": (_i32, [_bp, _u32, _u32, _u64, _u32, _u32]),
+    "lbm_unet_last_error": (ctypes.c_char_p, []),
+    "lbm_conv3x3_test": (_i32, [_i32, _i32, _i32, _i32, _i32, _i32, _fp, _fp, _fp, _fp, _fp, _i32]),
     "lbm_splitmix64": (_u64, [_u64p]),
     "lbm_gen_circles": (_i32, [_bp, _u32, _u32, _u64, _u32, _u32, _u32]),
 }
