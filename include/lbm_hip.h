@@ -238,6 +238,31 @@ int lbm_batch_read_moments(lbm_batch *b, uint32_t domain, float *jx, float *jy, 
 /* Text of the last error of an lbm_unet_* / lbm_conv3x3_test call on this thread. */
 const char *lbm_unet_last_error(void);
 
+/* A 4-level U-Net (3x3 convs, 32-64-128-256 channels, nearest-upsample + 1x1
+ * convs and skip concatenation on the way up, 1x1 head) with BF16 weights and
+ * activations and FP32 accumulation on the matrix cores, for `batch` images of
+ * height x width (height % 8 == 0, width % 256 == 0).  Input per pixel (NHWC,
+ * f32 on the host): solid mask, (rho - 1) / s_rho, ux / s_u, uy / s_u.  Output:
+ * the change of (rho, ux, uy) over one jump in the same units.  Parameters are
+ * f32 masters (weights [taps][cout][cin], then the bias, per layer). */
+typedef struct lbm_unet lbm_unet;
+int lbm_unet_create(int batch, int height, int width, uint64_t seed, int device, lbm_unet **out);
+int lbm_unet_destroy(lbm_unet *net);
+int lbm_unet_num_params(const lbm_unet *net, size_t *count);
+int lbm_unet_get_params(lbm_unet *net, float *params);
+int lbm_unet_set_params(lbm_unet *net, const float *params);
+/* x: batch*height*width*4 floats; y: batch*height*width*3 floats */
+int lbm_unet_forward(lbm_unet *net, const float *x, float *y);
+/* masked mean-square loss sum_p w_p |y_p - target_p|^2 / sum_p w_p and its
+ * gradient with respect to every parameter (grad may be NULL) */
+int lbm_unet_loss_grad(lbm_unet *net, const float *x, const float *target, const float *weights, float *loss,
+                       float *grad);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss; *loss = the loss before the step */
+int lbm_unet_train_step(lbm_unet *net, const float *x, const float *target, const float *weights, float lr,
+                        float *loss);
+/* device time of one forward pass (ms), averaged over reps */
+int lbm_unet_time_forward(lbm_unet *net, int reps, float *ms);
+
 /* The surrogate's 3x3 convolution primitives on host arrays, for testing: NHWC
  * f32 tensors rounded to bf16 on the device, f32 accumulation on the matrix
  * cores.  w is [9][Cout][Cin] (tap = 3 dy + dx).  mode 0: out (N,H,W,Cout) =

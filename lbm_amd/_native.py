@@ -116,6 +116,15 @@ SIGNATURES = {
     "lbm_gen_batch": (_i32, [_u64, _u32, ctypes.POINTER(lbm_domain)]),
     "lbm_gen_shapes": (_i32, [_bp, _u32, _u32, _u64, _u32, _u32]),
     "lbm_unet_last_error": (ctypes.c_char_p, []),
+    "lbm_unet_create": (_i32, [_i32, _i32, _i32, _u64, _i32, _PP]),
+    "lbm_unet_destroy": (_i32, [_P]),
+    "lbm_unet_num_params": (_i32, [_P, ctypes.POINTER(_sz)]),
+    "lbm_unet_get_params": (_i32, [_P, _fp]),
+    "lbm_unet_set_params": (_i32, [_P, _fp]),
+    "lbm_unet_forward": (_i32, [_P, _fp, _fp]),
+    "lbm_unet_loss_grad": (_i32, [_P, _fp, _fp, _fp, _fp, _fp]),
+    "lbm_unet_train_step": (_i32, [_P, _fp, _fp, _fp, _f32, _fp]),
+    "lbm_unet_time_forward": (_i32, [_P, _i32, _fp]),
     "lbm_conv3x3_test": (_i32, [_i32, _i32, _i32, _i32, _i32, _i32, _fp, _fp, _fp, _fp, _fp, _i32]),
     "lbm_splitmix64": (_u64, [_u64p]),
     "lbm_gen_circles": (_i32, [_bp, _u32, _u32, _u64, _u32, _u32, _u32]),
@@ -145,6 +154,13 @@ def load(path: str = LIB_PATH):
 def check(code: int, handle=None):
     if code != LBM_OK:
         msg = load().lbm_last_error(handle)
+        raise LbmError(code, msg.decode() if msg else "")
+    return code
+
+
+def check_unet(code: int):
+    if code != LBM_OK:
+        msg = load().lbm_unet_last_error()
         raise LbmError(code, msg.decode() if msg else "")
     return code
 
