@@ -263,6 +263,21 @@ int lbm_unet_train_step(lbm_unet *net, const float *x, const float *target, cons
 /* device time of one forward pass (ms), averaged over reps */
 int lbm_unet_time_forward(lbm_unet *net, int reps, float *ms);
 
+/* Surrogate jump on a grid (net shape (1, ny, nx), same device): predicts rho
+ * and u `steps` steps ahead from the current state with the scales s_rho, s_u
+ * of the network's units, writes the equilibrium of the prediction into every
+ * streamed fluid cell (solid cells, the inlet column and the last row keep their
+ * populations) and advances the step count by `steps`. */
+int lbm_surrogate_jump(lbm_grid *g, lbm_unet *net, uint32_t steps, float s_rho, float s_u);
+/* Device-side copy of the whole evolving state (one slot) and its restoration:
+ * the rollout's fallback. */
+int lbm_snapshot(lbm_grid *g);
+int lbm_restore(lbm_grid *g);
+/* Velocity residual: mark the current velocity field, then later
+ * rel = |u - u_mark| / |u_mark| over fluid cells (FP64 sums on the device). */
+int lbm_velocity_mark(lbm_grid *g);
+int lbm_velocity_change(lbm_grid *g, double *rel);
+
 /* The surrogate's 3x3 convolution primitives on host arrays, for testing: NHWC
  * f32 tensors rounded to bf16 on the device, f32 accumulation on the matrix
  * cores.  w is [9][Cout][Cin] (tap = 3 dy + dx).  mode 0: out (N,H,W,Cout) =

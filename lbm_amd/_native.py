@@ -125,6 +125,11 @@ SIGNATURES = {
     "lbm_unet_loss_grad": (_i32, [_P, _fp, _fp, _fp, _fp, _fp]),
     "lbm_unet_train_step": (_i32, [_P, _fp, _fp, _fp, _f32, _fp]),
     "lbm_unet_time_forward": (_i32, [_P, _i32, _fp]),
+    "lbm_surrogate_jump": (_i32, [_P, _P, _u32, _f32, _f32]),
+    "lbm_snapshot": (_i32, [_P]),
+    "lbm_restore": (_i32, [_P]),
+    "lbm_velocity_mark": (_i32, [_P]),
+    "lbm_velocity_change": (_i32, [_P, _dp]),
     "lbm_conv3x3_test": (_i32, [_i32, _i32, _i32, _i32, _i32, _i32, _fp, _fp, _fp, _fp, _fp, _i32]),
     "lbm_splitmix64": (_u64, [_u64p]),
     "lbm_gen_circles": (_i32, [_bp, _u32, _u32, _u64, _u32, _u32, _u32]),

@@ -97,7 +97,92 @@ __global__ __launch_bounds__(kBlock) void k_boundary(const typename St::T *src, 
     if (OP & OP_REDUCE) block_sums(s0, s1, s2, partials, blockIdx.x);
 }
 
+// ---- surrogate support ---------------------------------------------------------
+
+// network input (NHWC bf16, C channels) from the moments of the current state
+__global__ void k_pack_net_input(const float *__restrict__ mom, const uint8_t *__restrict__ mask,
+                                 uint16_t *__restrict__ xin, Geom g, int C, float inv_s_rho, float inv_s_u) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= g.X) return;
+    const size_t i = (size_t)y * g.pitch + x;
+    const bool solid = mask[i] == 1;
+    float v[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+    if (!solid) {
+        const float rho = mom[2 * g.pstride + i];
+        v[0] = 0.0f;
+        v[1] = (rho - 1.0f) * inv_s_rho;
+        v[2] = (mom[i] / rho) * inv_s_u;
+        v[3] = (mom[g.pstride + i] / rho) * inv_s_u;
+    }
+    uint16_t *px = xin + ((size_t)y * g.X + x) * C;
+    for (int c = 0; c < C; ++c) px[c] = c < 4 ? __builtin_bit_cast(uint16_t, (__bf16)v[c]) : (uint16_t)0;
+}
+
+// streamed fluid cells := feq(rho + s_rho d0, u + s_u (d1, d2)) in set `buf` (S-form)
+template <class St>
+__global__ void k_apply_jump(const float *__restrict__ net_out, int C, const float *__restrict__ mom,
+                             const uint8_t *__restrict__ mask, typename St::T *buf, Geom g, float s_rho, float s_u) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= g.X) return;
+    const size_t i = (size_t)y * g.pitch + x;
+    if (mask[i] == 1 || x == 0 || y >= g.Y - 1) return;
+    const float *o = net_out + ((size_t)y * g.X + x) * C;
+    const float rho0 = mom[2 * g.pstride + i];
+    const float rho = rho0 + s_rho * o[0];
+    const float ux = mom[i] / rho0 + s_u * o[1];
+    const float uym = -(mom[g.pstride + i] / rho0 + s_u * o[2]);  // memory rows grow southwards
+    const float uu = ux * ux + uym * uym;
+    const size_t sown = (size_t)y * g.ys + x;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const float cu = (float)cx(k) * ux + (float)cy(k) * uym;
+        const float feq = weight(k) * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
+        buf[(size_t)k * g.ks + sown] = St::from_f(feq, k);
+    }
+}
+
+// per-block FP64 sums of |u - u_mark|^2 and |u_mark|^2 over fluid cells
+__global__ __launch_bounds__(kBlock) void k_velocity_change(const float *__restrict__ mom,
+                                                            const float *__restrict__ mark,
+                                                            const uint8_t *__restrict__ mask, Geom g,
+                                                            double *__restrict__ partials) {
+    const int x = blockIdx.x * kBlock + threadIdx.x;
+    const int y = blockIdx.y;
+    double s0 = 0.0, s1 = 0.0;
+    if (x < g.X) {
+        const size_t i = (size_t)y * g.pitch + x;
+        if (mask[i] != 1) {
+            const float rho = mom[2 * g.pstride + i];
+            const double ux = mom[i] / rho, uy = mom[g.pstride + i] / rho;
+            const double mx = mark[i], my = mark[g.pstride + i];
+            s0 = (ux - mx) * (ux - mx) + (uy - my) * (uy - my);
+            s1 = mx * mx + my * my;
+        }
+    }
+    block_sums(s0, s1, 0.0, partials, (size_t)blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+__global__ void k_velocity_mark(const float *__restrict__ mom, float *__restrict__ mark, Geom g) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= g.X) return;
+    const size_t i = (size_t)y * g.pitch + x;
+    const float rho = mom[2 * g.pstride + i];
+    mark[i] = mom[i] / rho;
+    mark[g.pstride + i] = mom[g.pstride + i] / rho;
+}
+
 }  // namespace
+
+// provided by unet.hip (same library)
+struct lbm_unet;
+uint16_t *unet_input_tensor(lbm_unet *n);
+float *unet_output_tensor(lbm_unet *n);
+void unet_shape(const lbm_unet *n, int *b, int *h, int *w);
+void unet_forward_on(lbm_unet *n, hipStream_t s);
+int unet_device(const lbm_unet *n);
 
 struct lbm_grid {
     int dev = 0;
@@ -122,6 +207,15 @@ struct lbm_grid {
     uint64_t steps = 0;
     int variant = 0;  // step-kernel variant (cells per thread, non-temporal access)
     std::string err;
+    // device snapshot of the evolving state (both sets, fields, form, step count)
+    char *snap = nullptr;
+    bool snap_valid = false;
+    int snap_cur = 0;
+    bool snap_pform = false;
+    uint64_t snap_steps = 0;
+    float *umark = nullptr;  // velocity planes for the surrogate's residual
+    double *vpart = nullptr;
+    size_t nvpart = 0;
 
     size_t set_bytes() const { return (size_t)geom.ys * Y * esize; }
     struct Shape {
@@ -136,8 +230,12 @@ struct lbm_grid {
     void release() {
         (void)hipSetDevice(dev);
         for (void *p : {f[0], f[1], (void *)mask, (void *)flags, (void *)segs, (void *)slist, (void *)scount,
-                        (void *)lag, (void *)now, (void *)out, (void *)dense, (void *)partials, (void *)sums})
+                        (void *)lag, (void *)now, (void *)out, (void *)dense, (void *)partials, (void *)sums,
+                        (void *)snap, (void *)umark, (void *)vpart})
             if (p) (void)hipFree(p);
+        snap = nullptr;
+        umark = nullptr;
+        vpart = nullptr;
         f[0] = f[1] = nullptr;
         mask = flags = segs = nullptr;
         slist = scount = nullptr;
@@ -711,6 +809,89 @@ int lbm_autotune(lbm_grid *g, uint32_t steps, int *best, float *ms) {
         if (best) *best = v_best;
         if (ms)
             for (int v = 0; v < LBM_NUM_VARIANTS; ++v) ms[v] = t[v] / steps;
+    });
+}
+
+int lbm_snapshot(lbm_grid *g) {
+    return guarded(g, [&] {
+        const size_t sb = g->set_bytes(), lb = 3 * (size_t)g->geom.pstride * sizeof(float);
+        if (!g->snap) g->snap = dalloc<char>(2 * sb + lb);
+        HIPCHK(hipMemcpyAsync(g->snap, g->f[0], sb, hipMemcpyDeviceToDevice, g->stream));
+        HIPCHK(hipMemcpyAsync(g->snap + sb, g->f[1], sb, hipMemcpyDeviceToDevice, g->stream));
+        HIPCHK(hipMemcpyAsync(g->snap + 2 * sb, g->lag, lb, hipMemcpyDeviceToDevice, g->stream));
+        g->snap_cur = g->cur;
+        g->snap_pform = g->pform;
+        g->snap_steps = g->steps;
+        g->snap_valid = true;
+    });
+}
+
+int lbm_restore(lbm_grid *g) {
+    return guarded(g, [&] {
+        check(g->snap_valid, "no snapshot taken");
+        const size_t sb = g->set_bytes(), lb = 3 * (size_t)g->geom.pstride * sizeof(float);
+        HIPCHK(hipMemcpyAsync(g->f[0], g->snap, sb, hipMemcpyDeviceToDevice, g->stream));
+        HIPCHK(hipMemcpyAsync(g->f[1], g->snap + sb, sb, hipMemcpyDeviceToDevice, g->stream));
+        HIPCHK(hipMemcpyAsync(g->lag, g->snap + 2 * sb, lb, hipMemcpyDeviceToDevice, g->stream));
+        g->cur = g->snap_cur;
+        g->pform = g->snap_pform;
+        g->steps = g->snap_steps;
+    });
+}
+
+int lbm_velocity_mark(lbm_grid *g) {
+    return guarded(g, [&] {
+        if (!g->umark) g->umark = dalloc<float>(2 * (size_t)g->geom.pstride);
+        g->moments_now(false);
+        hipLaunchKernelGGL(k_velocity_mark, dim3((g->X + 255) / 256, g->Y), dim3(256), 0, g->stream, g->now,
+                           g->umark, g->geom);
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int lbm_velocity_change(lbm_grid *g, double *rel) {
+    return guarded(g, [&] {
+        check(rel != nullptr && g->umark != nullptr, "call lbm_velocity_mark first");
+        const dim3 grid((g->X + kBlock - 1) / kBlock, g->Y);
+        if (!g->vpart) {
+            g->nvpart = (size_t)grid.x * grid.y;
+            g->vpart = dalloc<double>(3 * g->nvpart);
+            if (!g->sums) g->sums = dalloc<double>(3);
+        }
+        g->moments_now(false);
+        hipLaunchKernelGGL(k_velocity_change, grid, dim3(kBlock), 0, g->stream, g->now, g->umark, g->mask, g->geom,
+                           g->vpart);
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, g->stream, g->vpart, g->nvpart, g->sums);
+        HIPCHK(hipGetLastError());
+        double s[3];
+        HIPCHK(hipMemcpyAsync(s, g->sums, sizeof s, hipMemcpyDeviceToHost, g->stream));
+        HIPCHK(hipStreamSynchronize(g->stream));
+        *rel = s[1] > 0.0 ? std::sqrt(s[0] / s[1]) : 0.0;
+    });
+}
+
+int lbm_surrogate_jump(lbm_grid *g, lbm_unet *net, uint32_t steps, float s_rho, float s_u) {
+    return guarded(g, [&] {
+        check(net != nullptr && s_rho > 0.0f && s_u > 0.0f, "bad network or scales");
+        int b, h, w;
+        unet_shape(net, &b, &h, &w);
+        check(b == 1 && h == (int)g->Y && w == (int)g->X, "network shape must be (1, ny, nx)");
+        check(unet_device(net) == g->dev, "network and grid on different devices");
+        g->flush();  // S-form: buf[cur] is the state to predict from and to overwrite
+        g->moments_now(false);
+        const dim3 grid((g->X + 255) / 256, g->Y);
+        hipLaunchKernelGGL(k_pack_net_input, grid, dim3(256), 0, g->stream, g->now, g->mask, unet_input_tensor(net),
+                           g->geom, 32, 1.0f / s_rho, 1.0f / s_u);
+        HIPCHK(hipGetLastError());
+        unet_forward_on(net, g->stream);
+        if (g->precision == LBM_PRECISION_F32)
+            hipLaunchKernelGGL(k_apply_jump<StoreF32>, grid, dim3(256), 0, g->stream, unet_output_tensor(net), 32,
+                               g->now, g->mask, (float *)g->f[g->cur], g->geom, s_rho, s_u);
+        else
+            hipLaunchKernelGGL(k_apply_jump<StoreF16S>, grid, dim3(256), 0, g->stream, unet_output_tensor(net), 32,
+                               g->now, g->mask, (uint16_t *)g->f[g->cur], g->geom, s_rho, s_u);
+        HIPCHK(hipGetLastError());
+        g->steps += steps;
     });
 }
 

@@ -240,6 +240,26 @@ class LBM:
         a = np.ascontiguousarray(np.asarray(f9, np.float32).reshape(9, self.y, self.x))
         self._chk(self._lib.lbm_write_f(self._h, _f32p(a)))
 
+    def snapshot(self):
+        """Device-side copy of the evolving state (one slot)."""
+        self._chk(self._lib.lbm_snapshot(self._h))
+
+    def restore(self):
+        self._chk(self._lib.lbm_restore(self._h))
+
+    def velocity_mark(self):
+        self._chk(self._lib.lbm_velocity_mark(self._h))
+
+    def velocity_change(self) -> float:
+        r = ctypes.c_double()
+        self._chk(self._lib.lbm_velocity_change(self._h, ctypes.byref(r)))
+        return r.value
+
+    def surrogate_jump(self, net, steps: int = 64, s_rho: float = 0.01, s_u: float = 0.1):
+        """Predict `steps` ahead with a trained UNet of shape (1, ny, nx) and
+        re-equilibrate the streamed fluid cells from the prediction."""
+        self._chk(self._lib.lbm_surrogate_jump(self._h, net._h, steps, s_rho, s_u))
+
     def save_state(self) -> bytes:
         n = ctypes.c_size_t()
         self._chk(self._lib.lbm_state_size(self._h, ctypes.byref(n)))

@@ -96,3 +96,139 @@ def test_mfma_conv_matches_oracle(N, H, W, Ci, Co):
     got = _run(2, N, H, W, Ci, Co, x, g, None, None).reshape(9, Co, Ci)
     ref = cr.conv3x3_wgrad(x, g)
     assert np.abs(got - ref).max() < 1e-4 * np.sqrt(N * H * W)
+
+
+def test_unet_reference_shapes_and_param_count():
+    from oracle import unet_ref
+    n = sum(k * k * ci * co + co for k, ci, co, _ in unet_ref.SPECS)
+    p = np.random.default_rng(0).standard_normal(n) * 0.05
+    y = unet_ref.forward(p, np.random.default_rng(1).standard_normal((1, 8, 32, 4)), round_bf16=False)
+    assert y.shape == (1, 8, 32, 3) and np.isfinite(y).all()
+
+
+def _sample_inputs(B, H, W, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((B, H, W, 4)).astype(np.float32)
+    x[..., 0] = (rng.random((B, H, W)) < 0.1).astype(np.float32)
+    t = rng.standard_normal((B, H, W, 3)).astype(np.float32) * 0.3
+    w = (x[..., 0] == 0).astype(np.float32)
+    return x, t, w
+
+
+@pytest.mark.gpu
+def test_unet_forward_matches_reference():
+    print("case: unet forward 1x16x256", flush=True)
+    from lbm_amd.surrogate import UNet
+    from oracle import unet_ref
+    net = UNet(1, 16, 256, seed=3)
+    p = net.get_params()
+    p[-(32 * 32 + 32):] *= 50.0  # a head of ordinary size, so the output is not ~0
+    net.set_params(p)
+    x, _, _ = _sample_inputs(1, 16, 256, 5)
+    got = net.forward(x)
+    ref = unet_ref.forward(p, cr.bf16(x), round_bf16=True)
+    scale = np.abs(ref).max()
+    assert scale > 1e-3
+    assert np.abs(got - ref).max() < 0.05 * scale
+    assert np.abs(got - ref).mean() < 0.01 * scale
+
+
+@pytest.mark.gpu
+def test_unet_gradients_match_finite_differences():
+    print("case: unet gradients 1x8x256", flush=True)
+    from lbm_amd.surrogate import UNet
+    from oracle import unet_ref
+    net = UNet(1, 8, 256, seed=11)
+    p = net.get_params().astype(np.float64)
+    p[-(32 * 32 + 32):] *= 50.0
+    net.set_params(p.astype(np.float32))
+    p = cr.bf16(p.astype(np.float32))  # the device uses bf16 weights
+    net.set_params(p.astype(np.float32))
+    x, t, w = _sample_inputs(1, 8, 256, 6)
+    x = cr.bf16(x).astype(np.float32)
+    loss, g = net.loss_grad(x, t, w)
+    assert abs(loss - unet_ref.loss(p, x, t, w)) < 0.05 * loss
+    rng = np.random.default_rng(0)
+    offs, picks = 0, []
+    for ks, ci, co, _ in unet_ref.SPECS:
+        nw = ks * ks * ci * co
+        live = np.arange(offs, offs + nw)
+        if offs == 0:  # first layer: only the 4 real input channels carry gradient
+            live = live[(live - offs) % ci < 4]
+        picks += list(rng.choice(live, 2, replace=False)) + [offs + nw + int(rng.integers(co if offs else co))]
+        offs += nw + co
+    picks = [i for i in picks if abs(g[i]) > 1e-6]
+    fd, an = [], []
+    for i in picks:
+        h = 1e-3 * max(1.0, abs(p[i]))
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd.append((unet_ref.loss(pp, x, t, w) - unet_ref.loss(pm, x, t, w)) / (2 * h))
+        an.append(g[i])
+    fd, an = np.array(fd), np.array(an)
+    cos = np.dot(fd, an) / (np.linalg.norm(fd) * np.linalg.norm(an))
+    rel = np.abs(fd - an) / np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max())
+    print(f"gradient check: {len(picks)} params, cosine {cos:.5f}, median rel err {np.median(rel):.3e}")
+    assert cos > 0.995
+    assert np.median(rel) < 0.05
+
+
+@pytest.mark.gpu
+def test_training_reduces_loss():
+    print("case: unet training", flush=True)
+    from lbm_amd.surrogate import UNet
+    net = UNet(2, 16, 256, seed=1)
+    x, _, w = _sample_inputs(2, 16, 256, 9)
+    t = 0.5 * x[..., 1:4] + 0.2 * np.roll(x[..., 1:4], 3, axis=2)  # a learnable local map
+    losses = [net.train_step(x, t, w, 2e-3) for _ in range(60)]
+    print(f"loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    assert losses[-1] < 0.3 * losses[0]
+
+
+@pytest.mark.gpu
+def test_rollout_forced_fallback_is_plain_lbm():
+    """threshold 0: every cycle falls back, so the rollout equals the plain LBM bit for bit."""
+    from lbm_amd import LBM, gen_circles
+    from lbm_amd.rollout import Rollout
+    from lbm_amd.surrogate import UNet
+    X, Y = 256, 64
+    mask = gen_circles(X, Y, 3, 4, 3, 8)
+    net = UNet(1, Y, X, seed=2)
+
+    def make():
+        g = LBM(1 / 0.56, X, Y)
+        g.mask = mask
+        g.custom_speed(0.08)
+        g.step(100)
+        return g
+    a, b = make(), make()
+    st = Rollout(a, net, threshold=0.0).run(72 * 3 + 10)
+    b.step(72 * 3 + 10)
+    assert st.fallbacks == 3 and st.jumps == 0
+    assert a.get_compute_num() == b.get_compute_num()
+    assert np.array_equal(a.read_f().view(np.uint32), b.read_f().view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_rollout_jump_path():
+    from lbm_amd import LBM, gen_circles
+    from lbm_amd.rollout import Rollout, velocity_error
+    from lbm_amd.surrogate import UNet
+    X, Y = 256, 64
+    mask = gen_circles(X, Y, 3, 4, 3, 8)
+    net = UNet(1, Y, X, seed=2)
+
+    def make():
+        g = LBM(1 / 0.56, X, Y)
+        g.mask = mask
+        g.custom_speed(0.08)
+        g.step(200)
+        return g
+    a, b = make(), make()
+    st = Rollout(a, net, threshold=float("inf")).run(72 * 4)
+    b.step(72 * 4)
+    assert st.jumps == 4 and st.fallbacks == 0 and a.get_compute_num() == b.get_compute_num()
+    err = velocity_error(a.fields(), b.fields(), mask)
+    print(f"untrained-network rollout error vs plain LBM: {err:.3e}; residuals {st.residuals}")
+    assert np.isfinite(err) and err < 1.0
