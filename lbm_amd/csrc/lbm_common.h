@@ -336,6 +336,114 @@ __device__ __forceinline__ void interior_tile(const typename St::T *src, typenam
     }
 }
 
+// One tile of a P-form pass with in-place patching, for any cells: the CPT
+// cells' pulls are loaded as vectors with clamped coordinates and the tile's
+// bounce-back bytes alongside; only cells whose byte is not 0 then have the
+// affected populations re-read one at a time:
+//   bit set  -> the cell's own opposite population (half-way bounce-back);
+//   0xFF     -> not streamed: the other set's own values (or, if streamed with
+//               all eight upstream cells solid, every population bounces back).
+// The first and last group of a row (inlet column, outlet column with its
+// west-moving wrap / clamp, and the clamped vector loads) are gathered cell by
+// cell.  The rest of the work is shared by all cells.
+template <class St, int MODE, int OP, int CPT, bool NT>
+__device__ __forceinline__ void patched_tile(const typename St::T *src, typename St::T *dst,
+                                             const uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask,
+                                             const Geom &g, int y, int x0, float *__restrict__ mom, double &s0,
+                                             double &s1, double &s2) {
+    using T = typename St::T;
+    using V = typename Vec<T, CPT>::a;
+    using U = typename Vec<T, CPT>::u;
+    using VF = typename Vec<float, CPT>::a;
+    const long long ps = g.pstride, ks = g.ks;
+    const size_t own0 = (size_t)y * g.pitch + x0;
+    const size_t sown0 = (size_t)y * g.ys + x0;
+    V v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int ry = min(max(y - cy(k), 0), g.Y - 1);
+        const int rx = min(max(x0 - cx(k), 0), g.pitch - CPT);
+        const size_t idx = (size_t)k * ks + (size_t)ry * g.ys + rx;
+        if (cx(k) == 0) v[k] = vload<V, NT>((const V *)(src + idx));
+        else v[k] = (V)vload<U, NT>((const U *)(src + idx));
+    }
+    uint64_t fw;
+    if constexpr (CPT == 8) fw = *(const uint64_t *)(flags + own0);
+    else if constexpr (CPT == 4) fw = *(const uint32_t *)(flags + own0);
+    else if constexpr (CPT == 2) fw = *(const uint16_t *)(flags + own0);
+    else fw = flags[own0];
+    // groups at either end of a row: the clamped vector loads above may be
+    // shifted by one element there, so every cell is gathered one by one
+    const bool edge = x0 == 0 || x0 + CPT >= g.X;
+    if (fw != 0 || edge) {
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int x = x0 + j;
+            if (x >= g.X) break;
+            const uint32_t fl = (uint32_t)(fw >> (8 * j)) & 0xFFu;
+            const size_t sown = sown0 + j;
+            if (edge) {
+                float f[9];
+                gather_cell<St, MODE, true>(src, dst, flags, mask, g, x, y, own0 + j, sown, f);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) set_lane<CPT>(v[k], j, St::store(f[k]));
+            } else if (fl == kSlowCell) {
+                const bool streamed = mask[own0 + j] != 1 && x != 0 && y < g.Y - 1;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    if (k == C) continue;
+                    set_lane<CPT>(v[k], j, streamed ? src[(size_t)opp(k) * ks + sown] : dst[(size_t)k * ks + sown]);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    if (k == C) continue;
+                    if ((fl >> bit(k)) & 1u) set_lane<CPT>(v[k], j, src[(size_t)opp(k) * ks + sown]);
+                }
+            }
+        }
+    }
+    V o[9];
+    VF mjx, mjy, mrho;
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        float f[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f[k] = St::load(get_lane<CPT>(v[k], j));
+        Moments m;
+        if (OP & OP_COLLIDE) {
+            m = St::collide(f, g.omega);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) set_lane<CPT>(o[k], j, St::store(f[k]));
+        } else {
+            m = St::moments(f, !(OP & OP_NOREST));
+        }
+        set_lane<CPT>(mjx, j, m.jx);
+        set_lane<CPT>(mjy, j, m.jy);
+        set_lane<CPT>(mrho, j, m.rho);
+        if ((OP & OP_REDUCE) && x0 + j < g.X && mask[own0 + j] != 1) {
+            s0 += (double)m.rho;
+            s1 += (double)m.jx;
+            s2 += (double)m.jy;
+        }
+    }
+    // whole vectors are stored; elements past the last column land in the row's
+    // padding, which holds no state
+    if (OP & OP_COLLIDE) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) vstore<V, NT>(o[k], (V *)(dst + (size_t)k * ks + sown0));
+    }
+    if (OP & OP_STORE_S) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) vstore<V, NT>(v[k], (V *)(dst + (size_t)k * ks + sown0));
+    }
+    if (OP & OP_MOMENTS) {
+        *(VF *)(mom + own0) = mjx;
+        *(VF *)(mom + ps + own0) = mjy;
+        *(VF *)(mom + 2 * ps + own0) = mrho;
+    }
+}
+
 template <class St>
 __global__ void k_fill(typename St::T *buf, Geom g, float4 v0123, float4 v4567, float v8) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

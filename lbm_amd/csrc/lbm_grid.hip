@@ -48,12 +48,13 @@
 
 namespace {
 
-// Interior kernel of one grid.  P-form: cells of clear segments only; S-form:
-// every cell (CPT = 1).
+// The step kernel of one grid: every cell, CPT consecutive cells per thread.
+// P-form: vector pulls patched in place where the bounce-back byte says so;
+// S-form (no pull): the cell's own values, CPT = 1.
 template <class St, int MODE, bool PFORM, int OP, int CPT, bool NT>
 __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, typename St::T *dst,
-                                                 const uint8_t *__restrict__ mask,
-                                                 const uint8_t *__restrict__ segs, Geom g,
+                                                 const uint8_t *__restrict__ flags,
+                                                 const uint8_t *__restrict__ mask, Geom g,
                                                  float *__restrict__ mom, double *__restrict__ partials,
                                                  int blocks_per_row, int nblocks, int remap) {
     static_assert(PFORM || CPT == 1, "S-form passes run one cell per thread");
@@ -61,38 +62,11 @@ __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, type
     const int y = L / blocks_per_row;
     const int x0 = ((L - y * blocks_per_row) * kBlock + threadIdx.x) * CPT;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    if (x0 < g.X)
-        interior_tile<St, MODE, PFORM, OP, CPT, NT, false>(src, dst, mask, segs, g, y, x0, mom, s0, s1, s2);
-    if (OP & OP_REDUCE) block_sums(s0, s1, s2, partials, blockIdx.x);
-}
-
-// Boundary kernel (P-form only): the listed segments, one wave per segment and
-// one cell per lane, with the general pull (bounce-back byte, outlet, cells
-// that are not streamed).  A fixed grid strides over the list, whose length is
-// read on the device.
-constexpr int kSlowBlocks = 1024;
-
-template <class St, int MODE, int OP>
-__global__ __launch_bounds__(kBlock) void k_boundary(const typename St::T *src, typename St::T *dst,
-                                                     const uint8_t *__restrict__ flags,
-                                                     const uint8_t *__restrict__ mask,
-                                                     const uint32_t *__restrict__ list,
-                                                     const uint32_t *__restrict__ count, Geom g,
-                                                     float *__restrict__ mom, double *__restrict__ partials) {
-    const uint32_t n = *count;
-    const int srow = g.pitch / 64 + kSegSlack;
-    const int lane = threadIdx.x & 63;
-    const uint32_t waves = gridDim.x * (kBlock / 64);
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (uint32_t e = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); e < n; e += waves) {
-        const uint32_t seg = list[e];
-        const int y = (int)(seg / (uint32_t)srow);
-        const int x = (int)(seg % (uint32_t)srow) * 64 + lane;
-        if (x >= g.X) continue;
-        const size_t own = (size_t)y * g.pitch + x, sown = (size_t)y * g.ys + x;
-        float f[9];
-        const bool streamed = gather_cell<St, MODE, true>(src, dst, flags, mask, g, x, y, own, sown, f);
-        finish_cell<St, OP>(f, streamed, src, dst, mask, g, own, sown, mom, s0, s1, s2);
+    if (x0 < g.X) {
+        if constexpr (PFORM)
+            patched_tile<St, MODE, OP, CPT, NT>(src, dst, flags, mask, g, y, x0, mom, s0, s1, s2);
+        else
+            interior_tile<St, MODE, false, OP, CPT, NT, true>(src, dst, mask, nullptr, g, y, x0, mom, s0, s1, s2);
     }
     if (OP & OP_REDUCE) block_sums(s0, s1, s2, partials, blockIdx.x);
 }
@@ -246,25 +220,17 @@ struct lbm_grid {
     }
 
     // ---- kernel launchers --------------------------------------------------
-    // One pass: the interior kernel, then (P-form) the boundary kernel.  They
-    // write disjoint cells and read only src and their own cells of dst.  FP64
-    // partial sums go to part[0, nblocks) and part[nblocks, nblocks + kSlowBlocks).
+    // One pass = one launch of the step kernel over every cell; FP64 partial
+    // sums (OP_REDUCE) go to part[0, nblocks).
     template <class St, int MODE, bool PFORM, int OP, int CPT, bool NT>
     void go(const void *src, void *dst, float *mom, double *part, int remap) {
         using T = typename St::T;
         const Shape sh = shape(CPT);
         hipLaunchKernelGGL((k_cell<St, MODE, PFORM, OP, CPT, NT>), dim3(sh.nblocks), dim3(kBlock), 0, stream,
-                           (const T *)src, (T *)dst, mask, segs, geom, mom, part, sh.blocks_per_row, sh.nblocks,
+                           (const T *)src, (T *)dst, flags, mask, geom, mom, part, sh.blocks_per_row, sh.nblocks,
                            remap);
         HIPCHK(hipGetLastError());
         last_partials = (size_t)sh.nblocks;
-        if constexpr (PFORM) {
-            hipLaunchKernelGGL((k_boundary<St, MODE, OP>), dim3(kSlowBlocks), dim3(kBlock), 0, stream,
-                               (const T *)src, (T *)dst, flags, mask, slist, scount, geom, mom,
-                               part ? part + 3 * (size_t)sh.nblocks : nullptr);
-            HIPCHK(hipGetLastError());
-            last_partials += kSlowBlocks;
-        }
     }
     size_t last_partials = 0;
 
@@ -352,7 +318,7 @@ struct lbm_grid {
     }
     void ensure_partials() {
         if (!partials) {
-            npartials = (size_t)shape(1).nblocks + kSlowBlocks;  // the largest pass
+            npartials = (size_t)shape(1).nblocks;  // the largest pass
             partials = dalloc<double>(3 * npartials);
             sums = dalloc<double>(3);
         }
