@@ -46,13 +46,16 @@ def read_traffic(name):
     return float(m.group(1)) if m else None
 
 
-def run_gpu(precision, X, Y, mask, steps, warmup, omega, u_in, tune=True):
+def run_gpu(precision, X, Y, mask, steps, warmup, omega, u_in, variant=None):
     from lbm_amd import LBM
     g = LBM(omega, X, Y, precision=precision)
     g.mask = mask
     g.custom_speed(u_in)
     tuned = None
-    if tune:
+    if variant is not None:
+        g.variant = variant
+        tuned = {"variant": variant, "us_per_step_by_variant": None}
+    else:
         best, ms = g.autotune(20)  # state is restored afterwards
         tuned = {"variant": best, "us_per_step_by_variant": [round(m * 1e3, 2) for m in ms]}
         log(f"autotune: variant {best}, us/step {tuned['us_per_step_by_variant']}")
@@ -104,6 +107,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f16", action="store_true")
+    ap.add_argument("--variant", type=int, default=None, help="FP32 kernel variant (default: autotune)")
+    ap.add_argument("--f16-variant", type=int, default=None, help="FP16-shifted kernel variant (default: autotune)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -121,7 +126,8 @@ def main():
     solid_frac = float(mask[1:-1].mean())
 
     log(f"f32: warmup {args.warmup}, timed {args.steps}")
-    wall, dev_ms, cons, tuned = run_gpu(PRECISION_F32, X, Y, mask, args.steps, args.warmup, omega, u_in)
+    wall, dev_ms, cons, tuned = run_gpu(PRECISION_F32, X, Y, mask, args.steps, args.warmup, omega, u_in,
+                                       args.variant)
     mlups = cells * args.steps / wall / 1e6
     kern_s = dev_ms / 1e3 / args.steps
     achieved = 72.0 * cells / kern_s / 1e9
@@ -163,7 +169,8 @@ def main():
 
     if not args.no_f16:
         log("f16s: same workload, FP16-shifted storage")
-        w16, d16, c16, tuned16 = run_gpu(PRECISION_F16S, X, Y, mask, args.steps, args.warmup, omega, u_in)
+        w16, d16, c16, tuned16 = run_gpu(PRECISION_F16S, X, Y, mask, args.steps, args.warmup, omega,
+                                        u_in, args.f16_variant)
         m16 = cells * args.steps / w16 / 1e6
         k16 = d16 / 1e3 / args.steps
         a16 = 36.0 * cells / k16 / 1e9
