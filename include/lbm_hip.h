@@ -242,16 +242,17 @@ const char *lbm_unet_last_error(void);
  * convs and skip concatenation on the way up, 1x1 head) with BF16 weights and
  * activations and FP32 accumulation on the matrix cores, for `batch` images of
  * height x width (height % 8 == 0, width % 256 == 0).  Input per pixel (NHWC,
- * f32 on the host): solid mask, (rho - 1) / s_rho, ux / s_u, uy / s_u.  Output:
- * the change of (rho, ux, uy) over one jump in the same units.  Parameters are
- * f32 masters (weights [taps][cout][cin], then the bias, per layer). */
+ * 8 f32 on the host): solid fraction, (rho - 1) / s_rho, ux / s_u, uy / s_u,
+ * (tau - 1/2) / 0.2, u_in / s_u, 0, 0.  Output: the change of (rho, ux, uy) over
+ * one jump in the same units.  Parameters are f32 masters (weights
+ * [taps][cout][cin], then the bias, per layer). */
 typedef struct lbm_unet lbm_unet;
 int lbm_unet_create(int batch, int height, int width, uint64_t seed, int device, lbm_unet **out);
 int lbm_unet_destroy(lbm_unet *net);
 int lbm_unet_num_params(const lbm_unet *net, size_t *count);
 int lbm_unet_get_params(lbm_unet *net, float *params);
 int lbm_unet_set_params(lbm_unet *net, const float *params);
-/* x: batch*height*width*4 floats; y: batch*height*width*3 floats */
+/* x: batch*height*width*8 floats; y: batch*height*width*3 floats */
 int lbm_unet_forward(lbm_unet *net, const float *x, float *y);
 /* masked mean-square loss sum_p w_p |y_p - target_p|^2 / sum_p w_p and its
  * gradient with respect to every parameter (grad may be NULL) */
@@ -263,11 +264,14 @@ int lbm_unet_train_step(lbm_unet *net, const float *x, const float *target, cons
 /* device time of one forward pass (ms), averaged over reps */
 int lbm_unet_time_forward(lbm_unet *net, int reps, float *ms);
 
-/* Surrogate jump on a grid (net shape (1, ny, nx), same device): predicts rho
- * and u `steps` steps ahead from the current state with the scales s_rho, s_u
- * of the network's units, writes the equilibrium of the prediction into every
- * streamed fluid cell (solid cells, the inlet column and the last row keep their
- * populations) and advances the step count by `steps`. */
+/* Surrogate jump on a grid (same device): the network predicts the change of
+ * rho and u over `steps` steps from the current state, in units s_rho, s_u;
+ * every streamed fluid cell (solid cells, the inlet column and the last row keep
+ * their populations) gets the equilibrium of its own rho, u plus that change,
+ * and the step count advances by `steps`.  Net shape (1, ny, nx): one network
+ * pixel per cell.  Net shape (1, ny / 2, nx / 2): each pixel sees a 2 x 2 block
+ * (solid fraction, fluid means of rho and u) and the change is bilinearly
+ * interpolated back to the cells. */
 int lbm_surrogate_jump(lbm_grid *g, lbm_unet *net, uint32_t steps, float s_rho, float s_u);
 /* Device-side copy of the whole evolving state (one slot) and its restoration:
  * the rollout's fallback. */
@@ -284,7 +288,13 @@ int lbm_velocity_change(lbm_grid *g, double *rel);
  * conv(in) + bias (bias may be NULL); mode 1: out (N,H,W,Cin) = input gradient
  * for the output gradient `in` (N,H,W,Cout); mode 2: out [9][Cout][Cin] = weight
  * gradient for input `in` and output gradient `g`.  W, Cin, Cout must be
- * multiples of 32. */
+ * multiples of 32.  mode 3: out[0] = milliseconds per launch of the mode-0
+ * convolution (bf16 output, ReLU), averaged over 20 launches.  Bits 4..7 of
+ * mode force the block shape of modes 0, 1 and 3 (0 = chosen by size; 1..13 =
+ * (rows per wave, 32-channel tiles per wave, waves along channels) of
+ * (1,1,1) (2,1,1) (4,1,1) (1,2,1) (2,2,1) (1,1,2) (2,1,2) (1,2,2) (2,2,2)
+ * (1,1,4) (2,1,4) (1,2,4) (2,2,4)), bits 8..10 the split of the input channels
+ * over blocks (0 = chosen by size, or 1, 2, 4). */
 int lbm_conv3x3_test(int mode, int n, int h, int w, int cin, int cout, const float *in, const float *g,
                      const float *weights, const float *bias, float *out, int device);
 

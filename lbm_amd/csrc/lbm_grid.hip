@@ -73,28 +73,55 @@ __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, type
 
 // ---- surrogate support ---------------------------------------------------------
 
-// network input (NHWC bf16, C channels) from the moments of the current state
+// Network input (NHWC bf16, C channels, nx / F x ny / F pixels) from the
+// moments of the current state.  Pixel (px, py) covers the F x F cells from
+// (F px, F py): channel 0 is their solid fraction, channels 1..3 the mean over
+// their fluid cells of (rho - 1) / s_rho, ux / s_u, uy / s_u (north-positive),
+// channels 4, 5 the grid's (tau - 1/2) / 0.2 and inflow speed u_in / s_u.
+template <int F>
 __global__ void k_pack_net_input(const float *__restrict__ mom, const uint8_t *__restrict__ mask,
-                                 uint16_t *__restrict__ xin, Geom g, int C, float inv_s_rho, float inv_s_u) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= g.X) return;
-    const size_t i = (size_t)y * g.pitch + x;
-    const bool solid = mask[i] == 1;
-    float v[4] = {1.0f, 0.0f, 0.0f, 0.0f};
-    if (!solid) {
-        const float rho = mom[2 * g.pstride + i];
-        v[0] = 0.0f;
-        v[1] = (rho - 1.0f) * inv_s_rho;
-        v[2] = (mom[i] / rho) * inv_s_u;
-        v[3] = (mom[g.pstride + i] / rho) * inv_s_u;
+                                 uint16_t *__restrict__ xin, Geom g, int C, float inv_s_rho, float inv_s_u,
+                                 float u_in) {
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    const int py = blockIdx.y;
+    const int w = g.X / F;
+    if (px >= w) return;
+    float nsolid = 0.0f, nfluid = 0.0f, srho = 0.0f, sux = 0.0f, suy = 0.0f;
+#pragma unroll
+    for (int j = 0; j < F; ++j)
+#pragma unroll
+        for (int k = 0; k < F; ++k) {
+            const size_t i = (size_t)(F * py + j) * g.pitch + F * px + k;
+            if (mask[i] == 1) {
+                nsolid += 1.0f;
+            } else {
+                const float rho = mom[2 * g.pstride + i];
+                nfluid += 1.0f;
+                srho += rho;
+                sux += mom[i] / rho;
+                suy += mom[g.pstride + i] / rho;
+            }
+        }
+    float v[6] = {nsolid * (1.0f / (F * F)), 0.0f, 0.0f, 0.0f, (1.0f / g.omega - 0.5f) * 5.0f, u_in * inv_s_u};
+    if (nfluid > 0.0f) {
+        const float r = 1.0f / nfluid;
+        v[1] = (srho * r - 1.0f) * inv_s_rho;
+        v[2] = sux * r * inv_s_u;
+        v[3] = suy * r * inv_s_u;
     }
-    uint16_t *px = xin + ((size_t)y * g.X + x) * C;
-    for (int c = 0; c < C; ++c) px[c] = c < 4 ? __builtin_bit_cast(uint16_t, (__bf16)v[c]) : (uint16_t)0;
+    typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+    u16x8 *out = (u16x8 *)(xin + ((size_t)py * w + px) * C);
+    u16x8 first = u16x8{};
+    for (int c = 0; c < 6; ++c) first[c] = __builtin_bit_cast(uint16_t, (__bf16)v[c]);
+    out[0] = first;
+    for (int c = 1; c < C / 8; ++c) out[c] = u16x8{};
 }
 
-// streamed fluid cells := feq(rho + s_rho d0, u + s_u (d1, d2)) in set `buf` (S-form)
-template <class St>
+// Streamed fluid cells := feq(rho + s_rho d0, u + s_u (d1, d2)) in set `buf`
+// (S-form), where (rho, u) is the cell's own current state and d the network's
+// predicted change, bilinearly interpolated from the nx / F x ny / F output
+// grid at the cell centre (exactly the cell's own pixel for F = 1).
+template <class St, int F>
 __global__ void k_apply_jump(const float *__restrict__ net_out, int C, const float *__restrict__ mom,
                              const uint8_t *__restrict__ mask, typename St::T *buf, Geom g, float s_rho, float s_u) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -102,11 +129,22 @@ __global__ void k_apply_jump(const float *__restrict__ net_out, int C, const flo
     if (x >= g.X) return;
     const size_t i = (size_t)y * g.pitch + x;
     if (mask[i] == 1 || x == 0 || y >= g.Y - 1) return;
-    const float *o = net_out + ((size_t)y * g.X + x) * C;
+    const int w = g.X / F, h = g.Y / F;
+    const float gx = fmaxf((x + 0.5f) / F - 0.5f, 0.0f), gy = fmaxf((y + 0.5f) / F - 0.5f, 0.0f);
+    const int i0 = min((int)gx, w - 1), j0 = min((int)gy, h - 1);
+    const int i1 = min(i0 + 1, w - 1), j1 = min(j0 + 1, h - 1);
+    const float fx = fminf(gx - (float)i0, 1.0f), fy = fminf(gy - (float)j0, 1.0f);
+    float d[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float o00 = net_out[((size_t)j0 * w + i0) * C + c], o01 = net_out[((size_t)j0 * w + i1) * C + c];
+        const float o10 = net_out[((size_t)j1 * w + i0) * C + c], o11 = net_out[((size_t)j1 * w + i1) * C + c];
+        d[c] = (1.0f - fy) * ((1.0f - fx) * o00 + fx * o01) + fy * ((1.0f - fx) * o10 + fx * o11);
+    }
     const float rho0 = mom[2 * g.pstride + i];
-    const float rho = rho0 + s_rho * o[0];
-    const float ux = mom[i] / rho0 + s_u * o[1];
-    const float uym = -(mom[g.pstride + i] / rho0 + s_u * o[2]);  // memory rows grow southwards
+    const float rho = rho0 + s_rho * d[0];
+    const float ux = mom[i] / rho0 + s_u * d[1];
+    const float uym = -(mom[g.pstride + i] / rho0 + s_u * d[2]);  // memory rows grow southwards
     const float uu = ux * ux + uym * uym;
     const size_t sown = (size_t)y * g.ys + x;
 #pragma unroll
@@ -166,6 +204,7 @@ struct lbm_grid {
     Geom geom{};
     size_t esize = 4;
     float omega = 1.25f;
+    float u_in = 0.1f;  // x velocity of the last uniform reset: the inlet column's speed
     void *f[2] = {nullptr, nullptr};
     int cur = 0;
     bool pform = false;
@@ -512,6 +551,7 @@ int lbm_create(const lbm_config *cfg, lbm_grid **out) {
         HIPCHK(hipMemsetAsync(g->out, 0, sizeof(float) * pstride, g->stream));
         float v[9];
         host_feq(cfg->u_init, 0.0f, 1.0f, v);
+        g->u_in = cfg->u_init;
         g->fill_both(v);
         HIPCHK(hipMemsetAsync(g->mask, 0, pstride, g->stream));
         hipLaunchKernelGGL(k_walls, dim3((g->X + 255) / 256), dim3(256), 0, g->stream, g->mask, g->geom);
@@ -559,6 +599,7 @@ int lbm_reset_equilibrium(lbm_grid *g, float rho, float ux, float uy) {
     return guarded(g, [&] {
         float v[9];
         host_feq(ux, uy, rho, v);
+        g->u_in = ux;
         g->fill_both(v);
         g->cur = 0;
         g->pform = false;
@@ -841,21 +882,34 @@ int lbm_surrogate_jump(lbm_grid *g, lbm_unet *net, uint32_t steps, float s_rho, 
         check(net != nullptr && s_rho > 0.0f && s_u > 0.0f, "bad network or scales");
         int b, h, w;
         unet_shape(net, &b, &h, &w);
-        check(b == 1 && h == (int)g->Y && w == (int)g->X, "network shape must be (1, ny, nx)");
+        const bool fine = b == 1 && h == (int)g->Y && w == (int)g->X;
+        const bool coarse = b == 1 && 2 * h == (int)g->Y && 2 * w == (int)g->X;
+        check(fine || coarse, "network shape must be (1, ny, nx) or (1, ny / 2, nx / 2)");
         check(unet_device(net) == g->dev, "network and grid on different devices");
         g->flush();  // S-form: buf[cur] is the state to predict from and to overwrite
         g->moments_now(false);
-        const dim3 grid((g->X + 255) / 256, g->Y);
-        hipLaunchKernelGGL(k_pack_net_input, grid, dim3(256), 0, g->stream, g->now, g->mask, unet_input_tensor(net),
-                           g->geom, 32, 1.0f / s_rho, 1.0f / s_u);
+        const dim3 pgrid((w + 255) / 256, h);
+        if (fine)
+            hipLaunchKernelGGL(k_pack_net_input<1>, pgrid, dim3(256), 0, g->stream, g->now, g->mask,
+                               unet_input_tensor(net), g->geom, 32, 1.0f / s_rho, 1.0f / s_u, g->u_in);
+        else
+            hipLaunchKernelGGL(k_pack_net_input<2>, pgrid, dim3(256), 0, g->stream, g->now, g->mask,
+                               unet_input_tensor(net), g->geom, 32, 1.0f / s_rho, 1.0f / s_u, g->u_in);
         HIPCHK(hipGetLastError());
         unet_forward_on(net, g->stream);
-        if (g->precision == LBM_PRECISION_F32)
-            hipLaunchKernelGGL(k_apply_jump<StoreF32>, grid, dim3(256), 0, g->stream, unet_output_tensor(net), 32,
-                               g->now, g->mask, (float *)g->f[g->cur], g->geom, s_rho, s_u);
-        else
-            hipLaunchKernelGGL(k_apply_jump<StoreF16S>, grid, dim3(256), 0, g->stream, unet_output_tensor(net), 32,
-                               g->now, g->mask, (uint16_t *)g->f[g->cur], g->geom, s_rho, s_u);
+        const dim3 grid((g->X + 255) / 256, g->Y);
+        const float *o = unet_output_tensor(net);
+#define APPLY(ST, TT, F)                                                                                          \
+    hipLaunchKernelGGL((k_apply_jump<ST, F>), grid, dim3(256), 0, g->stream, o, 32, g->now, g->mask, (TT *)g->f[g->cur], \
+                       g->geom, s_rho, s_u)
+        if (g->precision == LBM_PRECISION_F32) {
+            if (fine) APPLY(StoreF32, float, 1);
+            else APPLY(StoreF32, float, 2);
+        } else {
+            if (fine) APPLY(StoreF16S, uint16_t, 1);
+            else APPLY(StoreF16S, uint16_t, 2);
+        }
+#undef APPLY
         HIPCHK(hipGetLastError());
         g->steps += steps;
     });

@@ -34,63 +34,310 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
+// host-side input channels per pixel (solid fraction, (rho - 1) / s_rho, ux / s_u,
+// uy / s_u, (tau - 1/2) / 0.2, u_in / s_u, two spare zeros) and how many carry data
+constexpr int kNetIn = 8, kNetInLive = 6;
+
 __device__ __forceinline__ uint16_t f2bf(float v) {  // round to nearest even
     return __builtin_bit_cast(uint16_t, (__bf16)v);
 }
 __device__ __forceinline__ float bf2f(uint16_t v) { return (float)__builtin_bit_cast(__bf16, v); }
 
 // ---- convolution forward (and input gradient) ------------------------------------
-// in  [P = N*H*W][Cin] bf16, w [9][Cout][Cin] bf16, bias [Cout] f32 (may be null)
-// out [P][Cout] bf16 or f32.  Requirements: W % 32 == 0, Cin % 16 == 0,
-// Cout % (32 * NT) == 0.  A wave computes 32 pixels x (32 * NT) couts; a block
-// is four waves on consecutive 32-pixel runs.
-template <int KS, int NT, bool RELU, bool OUT_F32>
-__global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in, const uint16_t *__restrict__ w,
-                                                 const float *__restrict__ bias, void *__restrict__ out, int N, int H,
-                                                 int W, int Cin, int Cout) {
-    constexpr int TAPS = KS * KS, R = KS / 2;  // KS = 3: 3x3 with zero padding; KS = 1: 1x1
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+// Input of a convolution, NHWC bf16: channels [0, ca) come from a, channels
+// [ca, ca + cb) from b (cb may be 0).  With a_up, a has half the resolution and
+// is read through nearest 2x upsampling: the U-Net's "upsample, concatenate"
+// happens inside the convolution's loads and is never materialised.
+struct ConvIn {
+    const uint16_t *a = nullptr, *b = nullptr;
+    int ca = 0, cb = 0, a_up = 0;
+};
+
+typedef uint16_t u16x16 __attribute__((ext_vector_type(16)));  // 32 B: one lane's operands for two k-steps
+
+// out[p][co] = act(bias[co] + sum_{tap, ci} in[p + off(tap)][ci] w[tap][co][ci]),
+// w [taps][Cout][Cin] bf16.  Implicit GEMM, M = pixels, N = Cout, K = taps x Cin.
+// A wave computes MI runs of 32 pixels x NI tiles of 32 couts.  K advances in
+// chunks of 32 channels of one tap; lane half h loads channels 16h .. 16h + 15 of
+// the chunk as one 32-byte load, the first k-step consumes their first eight and
+// the second k-step their last eight (the same permutation of K for A and B).
+// Operands of the next chunk are loaded while the current one is multiplied.
+// SPLIT > 1: block z takes taps [z * 9 / SPLIT, (z + 1) * 9 / SPLIT) and writes
+// raw f32 sums to part[z][p][co]; k_conv_finish adds them up.
+// Requirements: W % 32 == 0, P % (32 * MI) == 0, ca % 32 == cb % 32 == 0,
+// Cout % (32 * NI) == 0.  Only channels co < co_store are stored.
+template <int KS, int MI, int NI, int SPLIT, bool RELU, bool OUT_F32>
+__global__ __launch_bounds__(256) void k_conv(ConvIn src, const uint16_t *__restrict__ w,
+                                              const float *__restrict__ bias, void *__restrict__ out,
+                                              float *__restrict__ part, int N, int H, int W, int Cout, int co_store) {
+    constexpr int TAPS = KS * KS, R = KS / 2, TPS = TAPS / SPLIT;
+    const int Cin = src.ca + src.cb;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int half = lane >> 5, l32 = lane & 31;
     const long long P = (long long)N * H * W;
-    const long long p0 = ((long long)blockIdx.x * 4 + wave) * 32;  // first pixel of this wave
+    const long long p0 = ((long long)blockIdx.x * 4 + wave) * (32 * MI);
     if (p0 >= P) return;
-    const int co0 = blockIdx.y * 32 * NT;
-    // the lane's A pixel
-    const long long pa = p0 + (lane & 31);
-    const int xa = (int)(pa % W);
-    const int ya = (int)((pa / W) % H);
-    const long long na = pa / ((long long)W * H);
-    const int kofs = 8 * (lane >> 5);
-    f32x16 acc[NT];
+    const int co0 = blockIdx.y * 32 * NI;
+    const int tap0 = blockIdx.z * TPS;
+    const int up = src.a_up ? 1 : 0;
+    const int Ha = H >> up, Wa = W >> up;
+    int ya[MI], xa[MI];
+    long long na[MI];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = f32x16{};
-    for (int tap = 0; tap < TAPS; ++tap) {
-        const int sy = ya + tap / KS - R, sx = xa + tap % KS - R;
-        const bool inside = sy >= 0 && sy < H && sx >= 0 && sx < W;
-        const uint16_t *arow = in + ((na * H + (inside ? sy : 0)) * W + (inside ? sx : 0)) * Cin + kofs;
-        const uint16_t *wrow = w + ((size_t)tap * Cout + co0 + (lane & 31)) * Cin + kofs;
-        for (int c0 = 0; c0 < Cin; c0 += 16) {
-            u16x8 av = inside ? *(const u16x8 *)(arow + c0) : u16x8{};
-            const bf16x8 a = __builtin_bit_cast(bf16x8, av);
+    for (int i = 0; i < MI; ++i) {
+        const long long p = p0 + 32 * i + l32;
+        xa[i] = (int)(p % W);
+        ya[i] = (int)((p / W) % H);
+        na[i] = p / ((long long)W * H);
+    }
+    const int nck = Cin >> 5;
+    const int nk = TPS * nck;
+    auto load = [&](int kk, u16x16(&a)[MI], u16x16(&b)[NI]) {
+        const int tap = tap0 + kk / nck;
+        const int c0 = ((kk % nck) << 5) + 16 * half;
+        const int dy = tap / KS - R, dx = tap % KS - R;
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const bf16x8 b = __builtin_bit_cast(bf16x8, *(const u16x8 *)(wrow + (size_t)t * 32 * Cin + c0));
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
+        for (int i = 0; i < MI; ++i) {
+            const int sy = ya[i] + dy, sx = xa[i] + dx;
+            const bool inside = sy >= 0 && sy < H && sx >= 0 && sx < W;
+            const int cy = inside ? sy : 0, cx = inside ? sx : 0;
+            const uint16_t *ptr = c0 < src.ca
+                ? src.a + ((na[i] * Ha + (cy >> up)) * Wa + (cx >> up)) * src.ca + c0
+                : src.b + ((na[i] * H + cy) * W + cx) * src.cb + (c0 - src.ca);
+            a[i] = inside ? *(const u16x16 *)ptr : u16x16{};
+        }
+#pragma unroll
+        for (int j = 0; j < NI; ++j)
+            b[j] = *(const u16x16 *)(w + ((size_t)tap * Cout + co0 + 32 * j + l32) * Cin + c0);
+    };
+    f32x16 acc[MI][NI];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NI; ++j) acc[i][j] = f32x16{};
+    auto mma = [&](const u16x16(&a)[MI], const u16x16(&b)[NI]) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+#pragma unroll
+            for (int i = 0; i < MI; ++i) {
+                const u16x8 a8 = s == 0 ? __builtin_shufflevector(a[i], a[i], 0, 1, 2, 3, 4, 5, 6, 7)
+                                        : __builtin_shufflevector(a[i], a[i], 8, 9, 10, 11, 12, 13, 14, 15);
+#pragma unroll
+                for (int j = 0; j < NI; ++j) {
+                    const u16x8 b8 = s == 0 ? __builtin_shufflevector(b[j], b[j], 0, 1, 2, 3, 4, 5, 6, 7)
+                                            : __builtin_shufflevector(b[j], b[j], 8, 9, 10, 11, 12, 13, 14, 15);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a8),
+                                                                        __builtin_bit_cast(bf16x8, b8), acc[i][j],
+                                                                        0, 0, 0);
+                }
+            }
+        }
+    };
+    u16x16 A0[MI], B0[NI], A1[MI], B1[NI];
+    load(0, A0, B0);
+    for (int kk = 0; kk < nk; kk += 2) {
+        if (kk + 1 < nk) load(kk + 1, A1, B1);
+        mma(A0, B0);
+        if (kk + 1 >= nk) break;
+        if (kk + 2 < nk) load(kk + 2, A0, B0);
+        mma(A1, B1);
+    }
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+            const int co = co0 + 32 * j + l32;
+            const float bv = (SPLIT == 1 && bias) ? bias[co] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const long long p = p0 + 32 * i + 8 * (r / 4) + 4 * half + (r % 4);
+                if (SPLIT > 1) {
+                    part[((long long)blockIdx.z * P + p) * Cout + co] = acc[i][j][r];
+                } else {
+                    float v = acc[i][j][r] + bv;
+                    if (RELU) v = fmaxf(v, 0.0f);
+                    if (co < co_store) {
+                        if (OUT_F32) ((float *)out)[p * Cout + co] = v;
+                        else ((uint16_t *)out)[p * Cout + co] = f2bf(v);
+                    }
+                }
             }
         }
     }
+}
+
+// LDS-staged 3x3 convolution.  A block computes TH = (4 / WN) * MI rows x 32
+// columns of output pixels of one image for 32 * WN * NI output channels; its
+// four waves form a (4 / WN) x WN grid, each computing MI rows x NI tiles of 32
+// channels.  For every 32-channel input chunk the block stages the chunk's
+// (TH + 2) x 34 pixel halo in LDS once (80-byte pixel stride: the 16-byte reads
+// of 16 consecutive pixels hit distinct banks) and runs all nine taps from it,
+// so each input value is fetched from L2 about (TH + 2) * 34 / (32 TH) times
+// instead of 9 * Cout / 32 times.  The next chunk's halo is loaded into registers
+// while the current one is multiplied (double-buffered LDS); weights are read
+// from global memory one tap ahead.  gridDim.z > 1 splits the input chunks
+// over blocks, which then write raw f32 sums to part[z][p][co] for
+// k_conv_finish.  Requirements: W % 32 == 0, ca % 32 == cb % 32 == 0,
+// Cout % (32 * WN * NI) == 0; H need not be a multiple of TH.
+template <int MI, int NI, int WN>
+__global__ __launch_bounds__(256) void k_conv3_lds(ConvIn src, const uint16_t *__restrict__ w,
+                                                   const float *__restrict__ bias, void *__restrict__ out,
+                                                   float *__restrict__ part, int N, int H, int W, int Cout,
+                                                   int co_store, int relu, int out_f32) {
+    constexpr int WM = 4 / WN, TH = WM * MI, HR = TH + 2, HC = 34, PS = 40;
+    constexpr int NPIECE = HR * HC * 4;  // 16-byte pieces of one chunk's halo
+    constexpr int PER_T = (NPIECE + 255) / 256;
+    __shared__ __align__(16) uint16_t lds[2][HR * HC * PS];
+    const int Cin = src.ca + src.cb;
+    const int tiles_x = W >> 5, tiles_y = (H + TH - 1) / TH;
+    int bid = blockIdx.x;
+    const int tx = bid % tiles_x;
+    bid /= tiles_x;
+    const int ty = bid % tiles_y;
+    const long long n = bid / tiles_y;
+    const int x0 = tx * 32, y0 = ty * TH;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int co0 = (blockIdx.y * WN + wn) * 32 * NI;
+    const int up = src.a_up ? 1 : 0, Ha = H >> up, Wa = W >> up;
+    const int nck = Cin >> 5;
+    const int ck0 = (int)(blockIdx.z * nck / gridDim.z), ck1 = (int)((blockIdx.z + 1) * nck / gridDim.z);
+
+    u16x8 stage[PER_T];
+    auto gload = [&](int ck) {
+        const int c0 = ck << 5;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int co = co0 + 32 * t + (lane & 31);
-        const float bv = bias ? bias[co] : 0.0f;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const long long p = p0 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
-            float v = acc[t][r] + bv;
-            if (RELU) v = fmaxf(v, 0.0f);
-            if (OUT_F32) ((float *)out)[p * Cout + co] = v;
-            else ((uint16_t *)out)[p * Cout + co] = f2bf(v);
+        for (int t = 0; t < PER_T; ++t) {
+            const int q = threadIdx.x + 256 * t;
+            stage[t] = u16x8{};
+            if (q < NPIECE) {
+                const int pi = q >> 2, c = c0 + ((q & 3) << 3);
+                const int y = y0 - 1 + pi / HC, x = x0 - 1 + pi % HC;
+                if (y >= 0 && y < H && x >= 0 && x < W) {
+                    const uint16_t *ptr = c < src.ca
+                        ? src.a + ((n * Ha + (y >> up)) * Wa + (x >> up)) * src.ca + c
+                        : src.b + ((n * H + y) * W + x) * src.cb + (c - src.ca);
+                    stage[t] = *(const u16x8 *)ptr;
+                }
+            }
         }
+    };
+    auto swrite = [&](int buf) {
+#pragma unroll
+        for (int t = 0; t < PER_T; ++t) {
+            const int q = threadIdx.x + 256 * t;
+            if (q < NPIECE) *(u16x8 *)&lds[buf][(q >> 2) * PS + ((q & 3) << 3)] = stage[t];
+        }
+    };
+    f32x16 acc[MI][NI];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NI; ++j) acc[i][j] = f32x16{};
+    // weights of one tap: lane's 8 channels 16 s + 8 half of the chunk, couts co0 + 32 j + l32
+    auto bload = [&](int tap, int c0, u16x8(&b)[NI][2]) {
+#pragma unroll
+        for (int j = 0; j < NI; ++j)
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2)
+                b[j][s2] = *(const u16x8 *)(w + ((size_t)tap * Cout + co0 + 32 * j + l32) * Cin + c0 + 16 * s2 +
+                                            8 * half);
+    };
+    auto compute = [&](int buf, int ck) {
+        const int c0 = ck << 5;
+        u16x8 bc[NI][2], bn[NI][2];
+        bload(0, c0, bc);
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            if (tap < 8) bload(tap + 1, c0, bn);
+            const int dy = tap / 3, dx = tap % 3;
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2) {
+#pragma unroll
+                for (int i = 0; i < MI; ++i) {
+                    const int r = wm * MI + i + dy;
+                    const u16x8 a = *(const u16x8 *)&lds[buf][(r * HC + l32 + dx) * PS + 16 * s2 + 8 * half];
+#pragma unroll
+                    for (int j = 0; j < NI; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            __builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, bc[j][s2]), acc[i][j], 0, 0, 0);
+                }
+            }
+            if (tap < 8) {
+#pragma unroll
+                for (int j = 0; j < NI; ++j) {
+                    bc[j][0] = bn[j][0];
+                    bc[j][1] = bn[j][1];
+                }
+            }
+        }
+    };
+    gload(ck0);
+    swrite(0);
+    __syncthreads();
+    for (int ck = ck0; ck < ck1; ++ck) {
+        const int buf = (ck - ck0) & 1;
+        if (ck + 1 < ck1) gload(ck + 1);
+        compute(buf, ck);
+        if (ck + 1 < ck1) swrite(buf ^ 1);
+        __syncthreads();
+    }
+    const long long P = (long long)N * H * W;
+    const bool split = gridDim.z > 1;
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+        const int y = y0 + wm * MI + i;
+        if (y >= H) continue;
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+            const int co = co0 + 32 * j + l32;
+            const float bv = (!split && bias) ? bias[co] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int x = x0 + 8 * (r / 4) + 4 * half + (r % 4);
+                const long long p = (n * H + y) * W + x;
+                if (split) {
+                    part[((long long)blockIdx.z * P + p) * Cout + co] = acc[i][j][r];
+                } else {
+                    float v = acc[i][j][r] + bv;
+                    if (relu) v = fmaxf(v, 0.0f);
+                    if (co < co_store) {
+                        if (out_f32) ((float *)out)[p * Cout + co] = v;
+                        else ((uint16_t *)out)[p * Cout + co] = f2bf(v);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// out = act(bias + sum of the split partials), four channels per thread
+template <bool RELU, bool OUT_F32>
+__global__ void k_conv_finish(const float *__restrict__ part, const float *__restrict__ bias, void *__restrict__ out,
+                              long long n, int Cout, int splits) {
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i >= n) return;
+    float4 s = *(const float4 *)(part + i);
+    for (int z = 1; z < splits; ++z) {
+        const float4 t = *(const float4 *)(part + (long long)z * n + i);
+        s.x += t.x;
+        s.y += t.y;
+        s.z += t.z;
+        s.w += t.w;
+    }
+    const int c = (int)(i % Cout);
+    float v[4] = {s.x, s.y, s.z, s.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] += bias ? bias[c + k] : 0.0f;
+        if (RELU) v[k] = fmaxf(v[k], 0.0f);
+    }
+    if (OUT_F32) {
+        *(float4 *)((float *)out + i) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+        const uint32_t lo = f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16), hi = f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+        *(uint2 *)((uint16_t *)out + i) = make_uint2(lo, hi);
     }
 }
 
@@ -100,10 +347,12 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ in
 // range of pixels split over its four waves; the four partial tiles are summed
 // in LDS and added to dw (f32) with vector atomics.
 template <int KS>
-__global__ __launch_bounds__(256) void k_conv3x3_wgrad(const uint16_t *__restrict__ in,
-                                                       const uint16_t *__restrict__ g, float *__restrict__ dw,
-                                                       int N, int H, int W, int Cin, int Cout, int pix_per_block) {
+__global__ __launch_bounds__(256) void k_conv3x3_wgrad(ConvIn src, const uint16_t *__restrict__ g,
+                                                       float *__restrict__ dw, int N, int H, int W, int Cout,
+                                                       int pix_per_block) {
     constexpr int R = KS / 2;
+    const int Cin = src.ca + src.cb;
+    const int up = src.a_up ? 1 : 0;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const long long P = (long long)N * H * W;
@@ -126,7 +375,11 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const uint16_t *__restric
                 const int x = (int)(p % W), y = (int)((p / W) % H);
                 const long long n = p / ((long long)W * H);
                 const int sy = y + dy, sx = x + dx;
-                if (sy >= 0 && sy < H && sx >= 0 && sx < W) iv = in[((n * H + sy) * W + sx) * Cin + ci0 + (lane & 31)];
+                if (sy >= 0 && sy < H && sx >= 0 && sx < W) {
+                    const int c = ci0 + (lane & 31);
+                    iv = c < src.ca ? src.a[((n * (H >> up) + (sy >> up)) * (W >> up) + (sx >> up)) * src.ca + c]
+                                    : src.b[((n * H + sy) * W + sx) * src.cb + (c - src.ca)];
+                }
             }
             av[j] = gv;
             bv[j] = iv;
@@ -193,9 +446,9 @@ __global__ void k_bias_grad(const uint16_t *__restrict__ g, float *__restrict__ 
     atomicAdd(&db[c], s);
 }
 
-// 2x2 average pool / its gradient, NHWC
+// 2x2 average pool, NHWC, eight channels per thread
 __global__ void k_pool2(const uint16_t *__restrict__ in, uint16_t *__restrict__ out, int N, int H, int W, int C) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
     const int Ho = H / 2, Wo = W / 2;
     if (i >= (size_t)N * Ho * Wo * C) return;
     const int c = (int)(i % C);
@@ -203,8 +456,41 @@ __global__ void k_pool2(const uint16_t *__restrict__ in, uint16_t *__restrict__ 
     const int x = (int)(q % Wo), y = (int)((q / Wo) % Ho);
     const size_t n = q / ((size_t)Wo * Ho);
     const size_t b = ((n * H + 2 * y) * W + 2 * x) * C + c;
-    const float s = (bf2f(in[b]) + bf2f(in[b + C])) + (bf2f(in[b + (size_t)W * C]) + bf2f(in[b + (size_t)W * C + C]));
-    out[i] = f2bf(0.25f * s);
+    const size_t r = (size_t)W * C;
+    const u16x8 v0 = *(const u16x8 *)(in + b), v1 = *(const u16x8 *)(in + b + C);
+    const u16x8 v2 = *(const u16x8 *)(in + b + r), v3 = *(const u16x8 *)(in + b + r + C);
+    u16x8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = f2bf(0.25f * ((bf2f(v0[k]) + bf2f(v1[k])) + (bf2f(v2[k]) + bf2f(v3[k]))));
+    *(u16x8 *)(out + i) = o;
+}
+
+// Gradient of a convolution input built as [up2(a) | b] (see ConvIn):
+// ga[q] = sum of gk's first ca channels over the 2x2 fine pixels of coarse pixel
+// q; gb[p] = gk's last cb channels.  H, W: the fine size; eight channels per thread.
+__global__ void k_split_up(const uint16_t *__restrict__ gk, uint16_t *__restrict__ ga, uint16_t *__restrict__ gb,
+                           int N, int H, int W, int ca, int cb) {
+    const size_t t = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
+    const size_t na = (size_t)N * (H / 2) * (W / 2) * ca, nb = (size_t)N * H * W * cb;
+    const int C = ca + cb;
+    if (t < na) {
+        const int c = (int)(t % ca);
+        const size_t q = t / ca;
+        const int x = (int)(q % (W / 2)), y = (int)((q / (W / 2)) % (H / 2));
+        const size_t n = q / ((size_t)(W / 2) * (H / 2));
+        const size_t b = ((n * H + 2 * y) * W + 2 * x) * C + c;
+        const size_t r = (size_t)W * C;
+        const u16x8 v0 = *(const u16x8 *)(gk + b), v1 = *(const u16x8 *)(gk + b + C);
+        const u16x8 v2 = *(const u16x8 *)(gk + b + r), v3 = *(const u16x8 *)(gk + b + r + C);
+        u16x8 o;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) o[k] = f2bf((bf2f(v0[k]) + bf2f(v1[k])) + (bf2f(v2[k]) + bf2f(v3[k])));
+        *(u16x8 *)(ga + t) = o;
+    } else if (t - na < nb) {
+        const size_t j = t - na;
+        const size_t p = j / cb;
+        *(u16x8 *)(gb + j) = *(const u16x8 *)(gk + p * C + ca + (j % cb));
+    }
 }
 
 // gin (+)= the pool gradient; H, W: the input (fine) size
@@ -243,61 +529,22 @@ __global__ void k_mse(const float *__restrict__ out, const float *__restrict__ t
     if ((threadIdx.x & 63) == 0) atomicAdd(loss, l * inv_wsum);
 }
 
-// network input from host fields: x[p] = (mask, (rho - 1) / s_rho, ux / s_u, uy / s_u, 0...)
-__global__ void k_pack_input(const float *__restrict__ fields4, uint16_t *__restrict__ x, long long P, int C) {
+// network input from host fields: x[p] = (the kNetIn host channels, then zeros)
+__global__ void k_pack_input(const float *__restrict__ fields, uint16_t *__restrict__ x, long long P, int C) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
-    for (int c = 0; c < C; ++c) x[p * C + c] = c < 4 ? f2bf(fields4[p * 4 + c]) : (uint16_t)0;
-}
-
-// nearest 2x upsample / its gradient, NHWC; H, W: the coarse size
-__global__ void k_up2(const uint16_t *__restrict__ in, uint16_t *__restrict__ out, int N, int H, int W, int C) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)N * 4 * H * W * C) return;
-    const int c = (int)(i % C);
-    const size_t q = i / C;
-    const int x = (int)(q % (2 * W)), y = (int)((q / (2 * W)) % (2 * H));
-    const size_t n = q / ((size_t)4 * W * H);
-    out[i] = in[((n * H + y / 2) * W + x / 2) * C + c];
-}
-
-__global__ void k_up2_bwd(const uint16_t *__restrict__ gout, uint16_t *__restrict__ gin, int N, int H, int W, int C) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)N * H * W * C) return;
-    const int c = (int)(i % C);
-    const size_t q = i / C;
-    const int x = (int)(q % W), y = (int)((q / W) % H);
-    const size_t n = q / ((size_t)W * H);
-    const size_t b = ((n * 2 * H + 2 * y) * 2 * W + 2 * x) * C + c;
-    const size_t r = (size_t)2 * W * C;
-    const float s = (bf2f(gout[b]) + bf2f(gout[b + C])) + (bf2f(gout[b + r]) + bf2f(gout[b + r + C]));
-    gin[i] = f2bf(s);
-}
-
-// out[p] = [a[p] (Ca channels), b[p] (Cb channels)]; and back
-__global__ void k_concat(const uint16_t *__restrict__ a, const uint16_t *__restrict__ b, uint16_t *__restrict__ out,
-                         long long P, int Ca, int Cb) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int C = Ca + Cb;
-    if (i >= (size_t)P * C) return;
-    const int c = (int)(i % C);
-    const size_t p = i / C;
-    out[i] = c < Ca ? a[p * Ca + c] : b[p * Cb + (c - Ca)];
-}
-
-__global__ void k_split(const uint16_t *__restrict__ in, uint16_t *__restrict__ a, uint16_t *__restrict__ b,
-                        long long P, int Ca, int Cb, int accumulate_b) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int C = Ca + Cb;
-    if (i >= (size_t)P * C) return;
-    const int c = (int)(i % C);
-    const size_t p = i / C;
-    if (c < Ca) {
-        a[p * Ca + c] = in[i];
-    } else {
-        const size_t j = p * Cb + (c - Ca);
-        b[j] = accumulate_b ? f2bf(bf2f(b[j]) + bf2f(in[i])) : in[i];
-    }
+    const float4 f0 = *(const float4 *)(fields + p * 8), f1 = *(const float4 *)(fields + p * 8 + 4);
+    u16x8 v;
+    v[0] = f2bf(f0.x);
+    v[1] = f2bf(f0.y);
+    v[2] = f2bf(f0.z);
+    v[3] = f2bf(f0.w);
+    v[4] = f2bf(f1.x);
+    v[5] = f2bf(f1.y);
+    v[6] = f2bf(f1.z);
+    v[7] = f2bf(f1.w);
+    *(u16x8 *)(x + p * C) = v;
+    for (int c = 8; c < C; c += 8) *(u16x8 *)(x + p * C + c) = u16x8{};
 }
 
 // Adam (Kingma and Ba 2015) on f32 master weights; refreshes the bf16 copy
@@ -344,43 +591,179 @@ void ucheck(bool ok, const char *msg) {
 
 inline unsigned nblk(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
-// launch the forward convolution with the widest cout tile that divides Cout
-template <int KS>
-void conv_fwd_ks(hipStream_t s, const uint16_t *in, const uint16_t *w, const float *bias, void *out, int N, int H,
-                 int W, int Cin, int Cout, bool relu, bool out_f32) {
-    const long long P = (long long)N * H * W;
-    const unsigned gx = (unsigned)((P + 127) / 128);
-#define LAUNCH(NT, R, F)                                                                                          \
-    hipLaunchKernelGGL((k_conv3x3<KS, NT, R, F>), dim3(gx, Cout / (32 * NT)), dim3(256), 0, s, in, w, bias, out, N, \
-                       H, W, Cin, Cout)
-    if (out_f32) {
-        LAUNCH(1, false, true);
-    } else if (Cout % 64 == 0) {
-        if (relu) LAUNCH(2, true, false);
-        else LAUNCH(2, false, false);
+ConvIn one_input(const uint16_t *a, int c) {
+    ConvIn s;
+    s.a = a;
+    s.ca = c;
+    return s;
+}
+
+// Block shapes of the LDS-staged 3x3 convolution: {MI, NI, WN}
+constexpr int kTiles3[13][3] = {{1, 1, 1}, {2, 1, 1}, {4, 1, 1}, {1, 2, 1}, {2, 2, 1}, {1, 1, 2}, {2, 1, 2},
+                                {1, 2, 2}, {2, 2, 2}, {1, 1, 4}, {2, 1, 4}, {1, 2, 4}, {2, 2, 4}};
+
+// Register tile / block shape and K split of a convolution launch.  tile: index
+// into kTiles3 plus one (0: choose); split: input-chunk splits (0: choose).
+struct ConvTile {
+    int tile = 0, split = 0;
+};
+
+struct ConvPlan {
+    int tile, split;
+};
+
+long long conv3_blocks(int t, int N, int H, int W, int Cout) {
+    const int th = (4 / kTiles3[t][2]) * kTiles3[t][0];
+    return (long long)N * ((H + th - 1) / th) * (W / 32) * (Cout / (32 * kTiles3[t][1] * kTiles3[t][2]));
+}
+
+// Block shape and split by layer shape, from timings of every shape on MI355X
+// for the network's layers (batch 1 at 512 x 1024 and batch 8 at 256 x 256):
+// few output channels favour short, single-wave-column tiles; wide layers
+// favour spreading the channels over the four waves.  A shape that leaves
+// fewer than 256 blocks (one per CU) falls back to a narrower one, then to a
+// split of the input chunks.
+ConvPlan conv3_plan(int N, int H, int W, int Cin, int Cout) {
+    // preference lists, indices into kTiles3
+    static const int p32a[] = {0, 1, 2};           // Cout 32, Cin 32
+    static const int p32b[] = {1, 2, 0};           // Cout 32, Cin > 32
+    static const int p64a[] = {5, 4, 0};           // Cout 64, Cin 32
+    static const int p64b[] = {4, 6, 1, 0};        // Cout 64, Cin > 32
+    static const int p128a[] = {10, 6, 1, 0};      // Cout 128, Cin <= 128
+    static const int p128b[] = {2, 10, 6, 1, 0};   // Cout 128, Cin > 128
+    static const int pwide[] = {10, 6, 1, 9, 0};   // Cout >= 256 (multiples of 128)
+    const int *list;
+    int len;
+    if (Cout == 32) {
+        list = Cin == 32 ? p32a : p32b;
+        len = 3;
+    } else if (Cout == 64) {
+        list = Cin == 32 ? p64a : p64b;
+        len = Cin == 32 ? 3 : 4;
+    } else if (Cout == 128) {
+        list = Cin <= 128 ? p128a : p128b;
+        len = Cin <= 128 ? 4 : 5;
+    } else if (Cout % 128 == 0) {
+        list = pwide;
+        len = 5;
     } else {
-        if (relu) LAUNCH(1, true, false);
-        else LAUNCH(1, false, false);
+        list = p64b;  // other multiples of 32 (64 k + 32): shapes with 32 channels per block column
+        len = 4;
     }
-#undef LAUNCH
+    int best = -1;
+    for (int i = 0; i < len && best < 0; ++i) {
+        const int t = list[i];
+        if (Cout % (32 * kTiles3[t][1] * kTiles3[t][2]) == 0 && conv3_blocks(t, N, H, W, Cout) >= 256) best = t;
+    }
+    if (best < 0) best = 0;
+    int split = 1;
+    const int nck = Cin / 32;
+    while (split < 4 && 2 * split <= nck && conv3_blocks(best, N, H, W, Cout) * split < 256) split *= 2;
+    return {best, split};
+}
+
+// f32 elements of split-K partial sums a 3x3 convolution needs (0: no split)
+size_t conv_split_elems(int ks, long long P, int N, int H, int W, int Cin, int Cout) {
+    if (ks != 3) return 0;
+    const ConvPlan pl = conv3_plan(N, H, W, Cin, Cout);
+    return pl.split > 1 ? (size_t)pl.split * P * Cout : 0;
+}
+
+// Launch a convolution (also used for input gradients): 3x3 on the LDS-staged
+// kernel, 1x1 on the register-tiled one.  Only channels co < co_store are
+// stored (all of them when the K split is taken).
+void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const float *bias, void *out, int N, int H,
+              int W, int Cout, bool relu, bool out_f32, int co_store, float *ws, size_t ws_elems,
+              ConvTile force = ConvTile{}) {
+    const long long P = (long long)N * H * W;
+    const int Cin = in.ca + in.cb;
+    ucheck(W % 32 == 0 && in.ca % 32 == 0 && in.cb % 32 == 0 && Cin > 0 && Cout % 32 == 0,
+           "convolution shape: W, channel counts must be multiples of 32");
+    ucheck(!in.a_up || H % 2 == 0, "upsampled input needs an even height");
+    if (ks == 1) {
+        ucheck(!in.a_up && in.cb == 0, "1x1 convolutions take one plain input");
+        int mi = 1, ni = 1;
+        auto waves = [&](int m, int n_) { return P / (32 * m) * (Cout / (32 * n_)); };
+        auto ok = [&](int m, int n_) { return P % (32 * m) == 0 && Cout % (32 * n_) == 0; };
+        const int cand[2][2] = {{2, 1}, {1, 2}};
+        for (auto &c : cand)
+            if (ok(c[0], c[1]) && waves(c[0], c[1]) >= 4096) {
+                mi = c[0];
+                ni = c[1];
+                break;
+            }
+        const dim3 grid((unsigned)((P + 128 * mi - 1) / (128 * mi)), (unsigned)(Cout / (32 * ni)), 1);
+#define KCONV(MI, NI, R, F)                                                                                        \
+    hipLaunchKernelGGL((k_conv<1, MI, NI, 1, R, F>), grid, dim3(256), 0, s, in, w, bias, out, nullptr, N, H, W, \
+                       Cout, co_store)
+#define PICK(R, F)                                                         \
+    do {                                                                   \
+        if (mi == 2) KCONV(2, 1, R, F);                                    \
+        else if (ni == 2) KCONV(1, 2, R, F);                               \
+        else KCONV(1, 1, R, F);                                            \
+    } while (0)
+        if (out_f32) PICK(false, true);
+        else if (relu) PICK(true, false);
+        else PICK(false, false);
+#undef PICK
+#undef KCONV
+        HIPCHK(hipGetLastError());
+        return;
+    }
+    ConvPlan pl = conv3_plan(N, H, W, Cin, Cout);
+    if (force.tile) pl.tile = force.tile - 1;
+    if (force.split) pl.split = force.split;
+    const int t = pl.tile;
+    ucheck(t >= 0 && t < 13 && Cout % (32 * kTiles3[t][1] * kTiles3[t][2]) == 0, "unsupported block shape");
+    ucheck(pl.split == 1 || pl.split == 2 || pl.split == 4, "unsupported split");
+    const bool split = pl.split > 1;
+    ucheck(!split || (ws && ws_elems >= (size_t)pl.split * P * Cout), "convolution split workspace too small");
+    const dim3 grid((unsigned)conv3_blocks(t, N, H, W, Cout) / (Cout / (32 * kTiles3[t][1] * kTiles3[t][2])),
+                    (unsigned)(Cout / (32 * kTiles3[t][1] * kTiles3[t][2])), (unsigned)pl.split);
+    const int r = relu ? 1 : 0, f = out_f32 ? 1 : 0;
+#define L3(MI, NI, WN)                                                                                          \
+    hipLaunchKernelGGL((k_conv3_lds<MI, NI, WN>), grid, dim3(256), 0, s, in, w, bias, out, ws, N, H, W, Cout, \
+                       co_store, r, f)
+    switch (t) {
+        case 0: L3(1, 1, 1); break;
+        case 1: L3(2, 1, 1); break;
+        case 2: L3(4, 1, 1); break;
+        case 3: L3(1, 2, 1); break;
+        case 4: L3(2, 2, 1); break;
+        case 5: L3(1, 1, 2); break;
+        case 6: L3(2, 1, 2); break;
+        case 7: L3(1, 2, 2); break;
+        case 8: L3(2, 2, 2); break;
+        case 9: L3(1, 1, 4); break;
+        case 10: L3(2, 1, 4); break;
+        case 11: L3(1, 2, 4); break;
+        default: L3(2, 2, 4); break;
+    }
+#undef L3
     HIPCHK(hipGetLastError());
+    if (split) {
+        const long long n = P * Cout;
+        const dim3 fg(nblk((size_t)(n / 4)));
+        if (out_f32)
+            hipLaunchKernelGGL((k_conv_finish<false, true>), fg, dim3(256), 0, s, ws, bias, out, n, Cout, pl.split);
+        else if (relu)
+            hipLaunchKernelGGL((k_conv_finish<true, false>), fg, dim3(256), 0, s, ws, bias, out, n, Cout, pl.split);
+        else
+            hipLaunchKernelGGL((k_conv_finish<false, false>), fg, dim3(256), 0, s, ws, bias, out, n, Cout, pl.split);
+        HIPCHK(hipGetLastError());
+    }
 }
 
-void conv_fwd(hipStream_t s, int ks, const uint16_t *in, const uint16_t *w, const float *bias, void *out, int N, int H,
-              int W, int Cin, int Cout, bool relu, bool out_f32) {
-    if (ks == 3) conv_fwd_ks<3>(s, in, w, bias, out, N, H, W, Cin, Cout, relu, out_f32);
-    else conv_fwd_ks<1>(s, in, w, bias, out, N, H, W, Cin, Cout, relu, out_f32);
-}
-
-void conv_wgrad(hipStream_t s, int ks, const uint16_t *in, const uint16_t *g, float *dw, int N, int H, int W, int Cin,
+void conv_wgrad(hipStream_t s, int ks, const ConvIn &in, const uint16_t *g, float *dw, int N, int H, int W,
                 int Cout) {
     const long long P = (long long)N * H * W;
+    const int Cin = in.ca + in.cb;
     const int ppb = 2048;
     const dim3 grid((unsigned)((P + ppb - 1) / ppb), (Cout / 32) * (Cin / 32), ks * ks);
     if (ks == 3)
-        hipLaunchKernelGGL(k_conv3x3_wgrad<3>, grid, dim3(256), 0, s, in, g, dw, N, H, W, Cin, Cout, ppb);
+        hipLaunchKernelGGL(k_conv3x3_wgrad<3>, grid, dim3(256), 0, s, in, g, dw, N, H, W, Cout, ppb);
     else
-        hipLaunchKernelGGL(k_conv3x3_wgrad<1>, grid, dim3(256), 0, s, in, g, dw, N, H, W, Cin, Cout, ppb);
+        hipLaunchKernelGGL(k_conv3x3_wgrad<1>, grid, dim3(256), 0, s, in, g, dw, N, H, W, Cout, ppb);
     HIPCHK(hipGetLastError());
 }
 
@@ -401,8 +784,8 @@ struct Conv {
 
 }  // namespace
 
-// The network.  Layers (c: 3x3 conv + ReLU, u: nearest 2x upsample then 1x1 conv +
-// ReLU, p: 2x2 average pool, [a|b]: channel concatenation):
+// The network.  Layers (c: 3x3 conv + ReLU, u: 1x1 conv + ReLU then nearest 2x
+// upsampling, p: 2x2 average pool, [a|b]: channel concatenation):
 //   e1 = c(c(x: 32))            32 ch, H      p1 = p(e1)
 //   e2 = c(c(p1))               64 ch, H/2    p2 = p(e2)
 //   e3 = c(c(p2))              128 ch, H/4    p3 = p(e3)
@@ -411,8 +794,14 @@ struct Conv {
 //   d2 = c(c([u(d3) | e2]))     64 ch, H/2
 //   d1 = c(c([u(d2) | e1]))     32 ch, H
 //   out = 1x1 conv(d1)          32 ch (the first 3 used), f32
-// Input channels: solid mask, (rho - 1) / s_rho, ux / s_u, uy / s_u, then zeros;
-// output: the 64-step change of (rho, ux, uy) in the same units.
+// A 1x1 convolution and ReLU act per pixel, so they commute with nearest
+// upsampling: u runs its 1x1 convolution at the coarse resolution (a quarter of
+// the work) and the upsampling and concatenation are folded into the next
+// convolution's loads (ConvIn).  Only the gradient of each concatenated input
+// (k3, k2, k1) is materialised, in the backward pass.
+// Input channels: solid fraction, (rho - 1) / s_rho, ux / s_u, uy / s_u,
+// (tau - 1/2) / 0.2, u_in / s_u, then zeros; output: the 64-step change of
+// (rho, ux, uy) in the same units.
 struct lbm_unet {
     int dev = 0;
     hipStream_t stream = nullptr;
@@ -423,8 +812,9 @@ struct lbm_unet {
     uint16_t *wb = nullptr, *wt = nullptr;
     float *out32 = nullptr, *tgt = nullptr, *wm = nullptr, *fields = nullptr, *loss = nullptr;
     uint16_t *gout = nullptr;
-    Act x, e1a, e1, p1, e2a, e2, p2, e3a, e3, p3, e4a, e4, u3, d3u, k3, d3a, d3, u2, d2u, k2, d2a, d2, u1, d1u, k1,
-        d1a, d1;
+    Act x, e1a, e1, p1, e2a, e2, p2, e3a, e3, p3, e4a, e4, d3u, k3, d3a, d3, d2u, k2, d2a, d2, d1u, k1, d1a, d1;
+    float *ws = nullptr;  // split-K partial sums
+    size_t ws_elems = 0;
     std::vector<void *> mem;
     int adam_t = 0;
 
@@ -433,12 +823,12 @@ struct lbm_unet {
         mem.push_back(p);
         return p;
     }
-    void make(Act &t, int h, int w, int c) {
+    void make(Act &t, int h, int w, int c, bool values = true) {
         t.N = B;
         t.H = h;
         t.W = w;
         t.C = c;
-        t.a = (uint16_t *)alloc(t.n() * 2);
+        t.a = values ? (uint16_t *)alloc(t.n() * 2) : nullptr;
         t.g = (uint16_t *)alloc(t.n() * 2);
     }
     void release() {
@@ -472,19 +862,16 @@ struct lbm_unet {
         make(p3, h8, w8, 128);
         make(e4a, h8, w8, 256);
         make(e4, h8, w8, 256);
-        make(u3, h4, w4, 256);
-        make(d3u, h4, w4, 128);
-        make(k3, h4, w4, 256);
+        make(d3u, h8, w8, 128);
+        make(k3, h4, w4, 256, false);
         make(d3a, h4, w4, 128);
         make(d3, h4, w4, 128);
-        make(u2, h2, w2, 128);
-        make(d2u, h2, w2, 64);
-        make(k2, h2, w2, 128);
+        make(d2u, h4, w4, 64);
+        make(k2, h2, w2, 128, false);
         make(d2a, h2, w2, 64);
         make(d2, h2, w2, 64);
-        make(u1, H, W, 64);
-        make(d1u, H, W, 32);
-        make(k1, H, W, 64);
+        make(d1u, h2, w2, 32);
+        make(k1, H, W, 64, false);
         make(d1a, H, W, 32);
         make(d1, H, W, 32);
         const int specs[18][4] = {{3, 32, 32, 1},  {3, 32, 32, 1},   {3, 32, 64, 1},   {3, 64, 64, 1},
@@ -504,8 +891,19 @@ struct lbm_unet {
         gout = (uint16_t *)alloc(P * 32 * 2);
         tgt = (float *)alloc(P * 3 * 4);
         wm = (float *)alloc(P * 4);
-        fields = (float *)alloc(P * 4 * 4);
+        fields = (float *)alloc(P * 8 * 4);
         loss = (float *)alloc(4);
+        // split-K workspace: the largest need over the forward and input-gradient convolutions
+        for (const Conv &c : convs) {
+            // level of the layer from its output (forward) / input (gradient) channel count
+            for (int lv = 0; lv < 4; ++lv) {
+                const int h = H >> lv, w = W >> lv;
+                const long long Pl = (long long)B * h * w;
+                ws_elems = std::max({ws_elems, conv_split_elems(c.ks, Pl, B, h, w, c.cin, c.cout),
+                                     conv_split_elems(c.ks, Pl, B, h, w, c.cout, c.cin)});
+            }
+        }
+        if (ws_elems) ws = (float *)alloc(ws_elems * 4);
         // He-normal weights (fan-in = taps * real input channels), zero biases; the
         // head starts near zero so the untrained network predicts "no change"
         std::vector<float> hw(nparam, 0.0f);
@@ -517,12 +915,12 @@ struct lbm_unet {
         };
         for (size_t i = 0; i < convs.size(); ++i) {
             const Conv &c = convs[i];
-            const int fan_in = c.ks * c.ks * (i == 0 ? 4 : c.cin);
+            const int fan_in = c.ks * c.ks * (i == 0 ? kNetInLive : c.cin);
             const float std_ = (i + 1 == convs.size() ? 0.01f : 1.0f) * (float)std::sqrt(2.0 / fan_in);
             for (int t = 0; t < c.ks * c.ks; ++t)
                 for (int co = 0; co < c.cout; ++co)
                     for (int ci = 0; ci < c.cin; ++ci) {
-                        const bool live = (i != 0 || ci < 4) && (i + 1 != convs.size() || co < 3);
+                        const bool live = (i != 0 || ci < kNetIn) && (i + 1 != convs.size() || co < 3);
                         hw[c.woff + ((size_t)t * c.cout + co) * c.cin + ci] = live ? std_ * normal() : 0.0f;
                     }
         }
@@ -544,18 +942,24 @@ struct lbm_unet {
         HIPCHK(hipGetLastError());
     }
 
-    void fwd_conv(int i, const Act &in, const Act &out) {
+    static ConvIn cat_up(const Act &coarse, const Act &fine) {
+        ConvIn s;
+        s.a = coarse.a;
+        s.ca = coarse.C;
+        s.a_up = 1;
+        s.b = fine.a;
+        s.cb = fine.C;
+        return s;
+    }
+    void conv(int i, const ConvIn &in, int h, int w, uint16_t *out) {
         const Conv &c = convs[i];
-        conv_fwd(stream, c.ks, in.a, wb + c.wboff, w32 + c.boff, out.a, B, in.H, in.W, c.cin, c.cout, c.relu, false);
+        ucheck(in.ca + in.cb == c.cin, "layer input channels");
+        conv_fwd(stream, c.ks, in, wb + c.wboff, w32 + c.boff, out, B, h, w, c.cout, c.relu, false, c.cout, ws,
+                 ws_elems);
     }
+    void fwd_conv(int i, const Act &in, const Act &out) { conv(i, one_input(in.a, in.C), in.H, in.W, out.a); }
     void pool(const Act &in, const Act &out) {
-        hipLaunchKernelGGL(k_pool2, dim3(nblk(out.n())), dim3(256), 0, stream, in.a, out.a, B, in.H, in.W, in.C);
-    }
-    void up(const Act &in, const Act &out) {
-        hipLaunchKernelGGL(k_up2, dim3(nblk(out.n())), dim3(256), 0, stream, in.a, out.a, B, in.H, in.W, in.C);
-    }
-    void cat(const Act &a, const Act &b, const Act &out) {
-        hipLaunchKernelGGL(k_concat, dim3(nblk(out.n())), dim3(256), 0, stream, a.a, b.a, out.a, out.P(), a.C, b.C);
+        hipLaunchKernelGGL(k_pool2, dim3(nblk(out.n() / 8)), dim3(256), 0, stream, in.a, out.a, B, in.H, in.W, in.C);
     }
 
     void forward() {
@@ -570,44 +974,42 @@ struct lbm_unet {
         pool(e3, p3);
         fwd_conv(6, p3, e4a);
         fwd_conv(7, e4a, e4);
-        up(e4, u3);
-        fwd_conv(8, u3, d3u);
-        cat(d3u, e3, k3);
-        fwd_conv(9, k3, d3a);
+        fwd_conv(8, e4, d3u);
+        conv(9, cat_up(d3u, e3), e3.H, e3.W, d3a.a);
         fwd_conv(10, d3a, d3);
-        up(d3, u2);
-        fwd_conv(11, u2, d2u);
-        cat(d2u, e2, k2);
-        fwd_conv(12, k2, d2a);
+        fwd_conv(11, d3, d2u);
+        conv(12, cat_up(d2u, e2), e2.H, e2.W, d2a.a);
         fwd_conv(13, d2a, d2);
-        up(d2, u1);
-        fwd_conv(14, u1, d1u);
-        cat(d1u, e1, k1);
-        fwd_conv(15, k1, d1a);
+        fwd_conv(14, d2, d1u);
+        conv(15, cat_up(d1u, e1), e1.H, e1.W, d1a.a);
         fwd_conv(16, d1a, d1);
         const Conv &h = convs[17];
-        conv_fwd(stream, h.ks, d1.a, wb + h.wboff, w32 + h.boff, out32, B, H, W, h.cin, h.cout, false, true);
+        conv_fwd(stream, h.ks, one_input(d1.a, d1.C), wb + h.wboff, w32 + h.boff, out32, B, H, W, h.cout, false, true,
+                 3, ws, ws_elems);
         HIPCHK(hipGetLastError());
     }
 
     // gradients of conv i: weights and bias into g32; input gradient into gin (if any)
-    void bwd_conv(int i, const Act &in, const uint16_t *gout_, const Act *gin) {
+    void bwd_conv(int i, const ConvIn &in, int h, int w, const uint16_t *gout_, uint16_t *gin) {
         const Conv &c = convs[i];
-        conv_wgrad(stream, c.ks, in.a, gout_, g32 + c.woff, B, in.H, in.W, c.cin, c.cout);
+        conv_wgrad(stream, c.ks, in, gout_, g32 + c.woff, B, h, w, c.cout);
         hipLaunchKernelGGL(k_bias_grad, dim3(256, (c.cout + 63) / 64), dim3(256), 0, stream, gout_, g32 + c.boff,
-                           in.P(), c.cout);
+                           (long long)B * h * w, c.cout);
         if (gin)
-            conv_fwd(stream, c.ks, gout_, wt + c.wboff, nullptr, gin->g, B, in.H, in.W, c.cout, c.cin, false, false);
+            conv_fwd(stream, c.ks, one_input(gout_, c.cout), wt + c.wboff, nullptr, gin, B, h, w, c.cin, false, false,
+                     c.cin, ws, ws_elems);
+    }
+    void bwd_conv(int i, const Act &in, const uint16_t *gout_, uint16_t *gin) {
+        bwd_conv(i, one_input(in.a, in.C), in.H, in.W, gout_, gin);
     }
     void relu_bwd(const Act &t) {
         hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(t.n())), dim3(256), 0, stream, t.g, t.a, t.n());
     }
-    void split(const Act &k, const Act &a, const Act &b) {
-        hipLaunchKernelGGL(k_split, dim3(nblk(k.n())), dim3(256), 0, stream, k.g, a.g, b.g, k.P(), a.C, b.C, 0);
-    }
-    void up_bwd(const Act &fine, const Act &coarse) {
-        hipLaunchKernelGGL(k_up2_bwd, dim3(nblk(coarse.n())), dim3(256), 0, stream, fine.g, coarse.g, B, coarse.H,
-                           coarse.W, coarse.C);
+    // gradient of [up2(coarse) | fine] -> coarse.g (assigned), fine.g (assigned)
+    void split_up(const Act &k, const Act &coarse, const Act &fine) {
+        const size_t n = coarse.n() + fine.n();
+        hipLaunchKernelGGL(k_split_up, dim3(nblk(n / 8)), dim3(256), 0, stream, k.g, coarse.g, fine.g, B, k.H, k.W,
+                           coarse.C, fine.C);
     }
     void pool_bwd(const Act &coarse, const Act &fine) {  // accumulates into fine.g
         hipLaunchKernelGGL(k_pool2_bwd, dim3(nblk(fine.n())), dim3(256), 0, stream, coarse.g, fine.g, B, fine.H,
@@ -616,48 +1018,45 @@ struct lbm_unet {
 
     void backward() {
         HIPCHK(hipMemsetAsync(g32, 0, nparam * 4, stream));
-        bwd_conv(17, d1, gout, &d1);
+        bwd_conv(17, d1, gout, d1.g);
         relu_bwd(d1);
-        bwd_conv(16, d1a, d1.g, &d1a);
+        bwd_conv(16, d1a, d1.g, d1a.g);
         relu_bwd(d1a);
-        bwd_conv(15, k1, d1a.g, &k1);
-        split(k1, d1u, e1);
+        bwd_conv(15, cat_up(d1u, e1), H, W, d1a.g, k1.g);
+        split_up(k1, d1u, e1);
         relu_bwd(d1u);
-        bwd_conv(14, u1, d1u.g, &u1);
-        up_bwd(u1, d2);
+        bwd_conv(14, d2, d1u.g, d2.g);
         relu_bwd(d2);
-        bwd_conv(13, d2a, d2.g, &d2a);
+        bwd_conv(13, d2a, d2.g, d2a.g);
         relu_bwd(d2a);
-        bwd_conv(12, k2, d2a.g, &k2);
-        split(k2, d2u, e2);
+        bwd_conv(12, cat_up(d2u, e2), e2.H, e2.W, d2a.g, k2.g);
+        split_up(k2, d2u, e2);
         relu_bwd(d2u);
-        bwd_conv(11, u2, d2u.g, &u2);
-        up_bwd(u2, d3);
+        bwd_conv(11, d3, d2u.g, d3.g);
         relu_bwd(d3);
-        bwd_conv(10, d3a, d3.g, &d3a);
+        bwd_conv(10, d3a, d3.g, d3a.g);
         relu_bwd(d3a);
-        bwd_conv(9, k3, d3a.g, &k3);
-        split(k3, d3u, e3);
+        bwd_conv(9, cat_up(d3u, e3), e3.H, e3.W, d3a.g, k3.g);
+        split_up(k3, d3u, e3);
         relu_bwd(d3u);
-        bwd_conv(8, u3, d3u.g, &u3);
-        up_bwd(u3, e4);
+        bwd_conv(8, e4, d3u.g, e4.g);
         relu_bwd(e4);
-        bwd_conv(7, e4a, e4.g, &e4a);
+        bwd_conv(7, e4a, e4.g, e4a.g);
         relu_bwd(e4a);
-        bwd_conv(6, p3, e4a.g, &p3);
+        bwd_conv(6, p3, e4a.g, p3.g);
         pool_bwd(p3, e3);
         relu_bwd(e3);
-        bwd_conv(5, e3a, e3.g, &e3a);
+        bwd_conv(5, e3a, e3.g, e3a.g);
         relu_bwd(e3a);
-        bwd_conv(4, p2, e3a.g, &p2);
+        bwd_conv(4, p2, e3a.g, p2.g);
         pool_bwd(p2, e2);
         relu_bwd(e2);
-        bwd_conv(3, e2a, e2.g, &e2a);
+        bwd_conv(3, e2a, e2.g, e2a.g);
         relu_bwd(e2a);
-        bwd_conv(2, p1, e2a.g, &p1);
+        bwd_conv(2, p1, e2a.g, p1.g);
         pool_bwd(p1, e1);
         relu_bwd(e1);
-        bwd_conv(1, e1a, e1.g, &e1a);
+        bwd_conv(1, e1a, e1.g, e1a.g);
         relu_bwd(e1a);
         bwd_conv(0, x, e1a.g, nullptr);
         HIPCHK(hipGetLastError());
@@ -666,7 +1065,7 @@ struct lbm_unet {
     // host inputs -> device; returns the loss
     float loss_and_grad(const float *hx, const float *hy, const float *hwm) {
         const size_t P = (size_t)B * H * W;
-        HIPCHK(hipMemcpyAsync(fields, hx, P * 16, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(fields, hx, P * 32, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(tgt, hy, P * 12, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(wm, hwm, P * 4, hipMemcpyHostToDevice, stream));
         double wsum = 0.0;
@@ -782,7 +1181,7 @@ int lbm_unet_forward(lbm_unet *n, const float *x, float *y) {
         ucheck(n && x && y, "null argument");
         HIPCHK(hipSetDevice(n->dev));
         const size_t P = (size_t)n->B * n->H * n->W;
-        HIPCHK(hipMemcpyAsync(n->fields, x, P * 16, hipMemcpyHostToDevice, n->stream));
+        HIPCHK(hipMemcpyAsync(n->fields, x, P * 32, hipMemcpyHostToDevice, n->stream));
         hipLaunchKernelGGL(k_pack_input, dim3(nblk(P)), dim3(256), 0, n->stream, n->fields, n->x.a, (long long)P, 32);
         n->forward();
         HIPCHK(hipMemcpy2DAsync(y, 12, n->out32, 128, 12, P, hipMemcpyDeviceToHost, n->stream));
@@ -840,7 +1239,13 @@ int lbm_unet_time_forward(lbm_unet *n, int reps, float *ms) {
 int lbm_conv3x3_test(int mode, int N, int H, int W, int Cin, int Cout, const float *in, const float *g,
                      const float *w, const float *bias, float *out, int device) {
     return uguard([&] {
-        ucheck(mode >= 0 && mode <= 2, "mode");
+        const int op = mode & 3;
+        ConvTile force;
+        force.tile = (mode >> 4) & 15;
+        force.split = (mode >> 8) & 7;
+        ucheck(force.tile <= 13 && (mode >> 11) == 0, "mode");
+        ucheck(force.split == 0 || force.split == 1 || force.split == 2 || force.split == 4, "mode: split");
+        mode = op;
         ucheck(W % 32 == 0 && Cin % 32 == 0 && Cout % 32 == 0, "need W, Cin, Cout multiples of 32");
         HIPCHK(hipSetDevice(device));
         const size_t P = (size_t)N * H * W;
@@ -864,25 +1269,45 @@ int lbm_conv3x3_test(int mode, int N, int H, int W, int Cin, int Cout, const flo
                 hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(nw)), dim3(256), 0, 0, d32, dw, nw);
             }
             float *dout = (float *)alloc(std::max(P * cout_out, nw) * sizeof(float));
-            if (mode == 0) {
+            const size_t wsn = (size_t)4 * P * std::max(Cin, Cout);
+            float *ws = (float *)alloc(wsn * sizeof(float));
+            if (mode == 3) {  // time the forward convolution with a bf16 output, as the network runs it
+                hipEvent_t e0, e1;
+                HIPCHK(hipEventCreate(&e0));
+                HIPCHK(hipEventCreate(&e1));
+                const ConvIn ci = one_input(din, Cin);
+                for (int r = 0; r < 3; ++r)
+                    conv_fwd(0, 3, ci, dw, nullptr, dout, N, H, W, Cout, true, false, Cout, ws, wsn, force);
+                HIPCHK(hipEventRecord(e0, 0));
+                for (int r = 0; r < 20; ++r)
+                    conv_fwd(0, 3, ci, dw, nullptr, dout, N, H, W, Cout, true, false, Cout, ws, wsn, force);
+                HIPCHK(hipEventRecord(e1, 0));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                out[0] = ms / 20.0f;
+            } else if (mode == 0) {
                 float *db = nullptr;
                 if (bias) {
                     db = (float *)alloc(Cout * sizeof(float));
                     HIPCHK(hipMemcpy(db, bias, Cout * sizeof(float), hipMemcpyHostToDevice));
                 }
-                conv_fwd(0, 3, din, dw, db, dout, N, H, W, Cin, Cout, false, true);
+                conv_fwd(0, 3, one_input(din, Cin), dw, db, dout, N, H, W, Cout, false, true, Cout, ws, wsn, force);
                 HIPCHK(hipMemcpy(out, dout, P * Cout * sizeof(float), hipMemcpyDeviceToHost));
             } else if (mode == 1) {
                 uint16_t *dwt = (uint16_t *)alloc(nw * 2);
                 hipLaunchKernelGGL(k_flip_transpose, dim3(nblk(nw)), dim3(256), 0, 0, dw, dwt, Cin, Cout, 9);
-                conv_fwd(0, 3, din, dwt, nullptr, dout, N, H, W, Cout, Cin, false, true);
+                conv_fwd(0, 3, one_input(din, Cout), dwt, nullptr, dout, N, H, W, Cin, false, true, Cin, ws, wsn,
+                         force);
                 HIPCHK(hipMemcpy(out, dout, P * Cin * sizeof(float), hipMemcpyDeviceToHost));
             } else {
                 uint16_t *dg = (uint16_t *)alloc(P * Cout * 2);
                 HIPCHK(hipMemcpy(d32, g, P * Cout * sizeof(float), hipMemcpyHostToDevice));
                 hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(P * Cout)), dim3(256), 0, 0, d32, dg, P * Cout);
                 HIPCHK(hipMemset(dout, 0, nw * sizeof(float)));
-                conv_wgrad(0, 3, din, dg, dout, N, H, W, Cin, Cout);
+                conv_wgrad(0, 3, one_input(din, Cin), dg, dout, N, H, W, Cout);
                 HIPCHK(hipMemcpy(out, dout, nw * sizeof(float), hipMemcpyDeviceToHost));
             }
             HIPCHK(hipDeviceSynchronize());

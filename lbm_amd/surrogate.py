@@ -2,9 +2,10 @@
 
 ``UNet`` wraps the library's BF16-MFMA U-Net (lbm_unet_* in include/lbm_hip.h):
 forward, masked MSE loss + gradients, Adam training steps.  ``fields_to_input``
-and ``target_from`` define the normalised channels; ``Rollout`` (in
-rollout.py) uses a trained network to jump a grid 64 steps at a time with
-plain LBM corrector steps and a residual-gated fallback.
+and ``target_from`` define the normalised channels, pooled over F x F cells for
+a network that runs at 1/F of the grid's resolution (F = 1 or 2);
+``Rollout`` (in rollout.py) uses a trained network to jump a grid 64 steps at a
+time with plain LBM corrector steps and a residual-gated fallback.
 """
 from __future__ import annotations
 
@@ -16,29 +17,58 @@ from . import _native as N
 
 S_RHO = 0.01   # density deviation scale
 S_U = 0.1      # velocity scale
+NET_IN = 8     # host input channels per pixel
 
 
 def _fp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
-def fields_to_input(mask, rho, ux, uy, s_rho: float = S_RHO, s_u: float = S_U) -> np.ndarray:
-    """(..., H, W) fields -> (..., H, W, 4) network input; solid cells read as rest."""
-    solid = np.asarray(mask) == 1
-    x = np.stack([solid.astype(np.float32),
-                  np.where(solid, 0.0, (rho - 1.0) / s_rho),
-                  np.where(solid, 0.0, ux / s_u),
-                  np.where(solid, 0.0, uy / s_u)], axis=-1)
-    return np.ascontiguousarray(x, np.float32)
+def pool_fields(mask, rho, ux, uy, factor: int = 1):
+    """(..., H, W) cell fields -> (..., H/F, W/F) pixels of F x F cells: solid
+    fraction and the fluid-cell means of rho, ux, uy (0 where a pixel has no
+    fluid), as the grid's surrogate input kernel computes them."""
+    fl = (np.asarray(mask) != 1)
+    F = factor
+
+    def blocks(a):
+        a = np.asarray(a, np.float64)
+        sh = a.shape[:-2] + (a.shape[-2] // F, F, a.shape[-1] // F, F)
+        return a.reshape(sh).sum(axis=(-3, -1))
+    nf = blocks(fl)
+    safe = np.maximum(nf, 1.0)
+    means = [np.where(nf > 0, blocks(np.where(fl, v, 0.0)) / safe, 0.0) for v in (rho, ux, uy)]
+    return 1.0 - nf / (F * F), nf, means
 
 
-def target_from(mask, f0, f1, s_rho: float = S_RHO, s_u: float = S_U) -> np.ndarray:
-    """Change of (rho, ux, uy) between two field triples, in network units."""
-    solid = np.asarray(mask) == 1
-    d = [np.where(solid, 0.0, (f1[0] - f0[0]) / s_rho),
-         np.where(solid, 0.0, (f1[1] - f0[1]) / s_u),
-         np.where(solid, 0.0, (f1[2] - f0[2]) / s_u)]
+def fields_to_input(mask, rho, ux, uy, tau: float, u_in: float, factor: int = 1, s_rho: float = S_RHO,
+                    s_u: float = S_U) -> np.ndarray:
+    """(..., H, W) fields -> (..., H/F, W/F, NET_IN) network input."""
+    solid, nf, (r, u, v) = pool_fields(mask, rho, ux, uy, factor)
+    has = nf > 0
+    x = np.zeros(solid.shape + (NET_IN,), np.float32)
+    x[..., 0] = solid
+    x[..., 1] = np.where(has, (r - 1.0) / s_rho, 0.0)
+    x[..., 2] = np.where(has, u / s_u, 0.0)
+    x[..., 3] = np.where(has, v / s_u, 0.0)
+    x[..., 4] = (tau - 0.5) / 0.2
+    x[..., 5] = u_in / s_u
+    return x
+
+
+def target_from(mask, f0, f1, factor: int = 1, s_rho: float = S_RHO, s_u: float = S_U) -> np.ndarray:
+    """Change of the pooled (rho, ux, uy) between two field triples, in network units."""
+    _, nf, m0 = pool_fields(mask, *f0, factor)
+    _, _, m1 = pool_fields(mask, *f1, factor)
+    has = nf > 0
+    d = [np.where(has, (m1[i] - m0[i]) / s, 0.0) for i, s in enumerate((s_rho, s_u, s_u))]
     return np.ascontiguousarray(np.stack(d, axis=-1), np.float32)
+
+
+def loss_weights(mask, factor: int = 1) -> np.ndarray:
+    """Per-pixel loss weight: the fluid fraction of the pixel."""
+    solid, _, _ = pool_fields(mask, mask, mask, mask, factor)
+    return (1.0 - solid).astype(np.float32)
 
 
 class UNet:
@@ -82,13 +112,13 @@ class UNet:
         return a
 
     def forward(self, x) -> np.ndarray:
-        x = self._shape(x, 4)
+        x = self._shape(x, NET_IN)
         y = np.empty((self.batch, self.height, self.width, 3), np.float32)
         N.check_unet(self._lib.lbm_unet_forward(self._h, _fp(x), _fp(y)))
         return y
 
     def loss_grad(self, x, target, weights):
-        x, t = self._shape(x, 4), self._shape(target, 3)
+        x, t = self._shape(x, NET_IN), self._shape(target, 3)
         w = np.ascontiguousarray(weights, np.float32)
         loss = ctypes.c_float()
         g = np.empty(self.num_params, np.float32)
@@ -96,7 +126,7 @@ class UNet:
         return loss.value, g
 
     def train_step(self, x, target, weights, lr: float) -> float:
-        x, t = self._shape(x, 4), self._shape(target, 3)
+        x, t = self._shape(x, NET_IN), self._shape(target, 3)
         w = np.ascontiguousarray(weights, np.float32)
         loss = ctypes.c_float()
         N.check_unet(self._lib.lbm_unet_train_step(self._h, _fp(x), _fp(t), _fp(w), lr, ctypes.byref(loss)))

@@ -42,13 +42,13 @@ def _conv(x, layer, rnd):
     return y
 
 
-def forward(params, x4, round_bf16=True):
-    """x4 (N, H, W, 4) -> (N, H, W, 3)."""
+def forward(params, xin, round_bf16=True):
+    """xin (N, H, W, c <= 32) -> (N, H, W, 3); the input is zero-padded to 32 channels."""
     L = unpack(np.asarray(params, np.float64))
     r = (lambda a: bf16(a)) if round_bf16 else (lambda a: a)
-    N, H, W, _ = x4.shape
+    N, H, W, c = xin.shape
     x = np.zeros((N, H, W, 32))
-    x[..., :4] = x4
+    x[..., :c] = xin
     x = r(x)
 
     def c(i, t):
@@ -71,7 +71,7 @@ def forward(params, x4, round_bf16=True):
     return out[..., :3]
 
 
-def loss(params, x4, target, weights, round_bf16=False):
-    y = forward(params, x4, round_bf16)
+def loss(params, xin, target, weights, round_bf16=False):
+    y = forward(params, xin, round_bf16)
     w = np.asarray(weights, np.float64)
     return float((w[..., None] * (y - target) ** 2).sum() / w.sum())

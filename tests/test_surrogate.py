@@ -64,7 +64,7 @@ def _run(mode, N, H, W, Ci, Co, x, g, w, b):
     lib = Nn.load()
     fp = lambda a: None if a is None else np.ascontiguousarray(a, np.float32).ctypes.data_as(  # noqa: E731
         ctypes.POINTER(ctypes.c_float))
-    size = {0: N * H * W * Co, 1: N * H * W * Ci, 2: 9 * Co * Ci}[mode]
+    size = {0: N * H * W * Co, 1: N * H * W * Ci, 2: 9 * Co * Ci}[mode & 3]
     out = np.zeros(size, np.float32)
     keep = [np.ascontiguousarray(a, np.float32) if a is not None else None for a in (x, g, w, b)]
     rc = lib.lbm_conv3x3_test(mode, N, H, W, Ci, Co, *[fp(a) for a in keep],
@@ -73,9 +73,14 @@ def _run(mode, N, H, W, Ci, Co, x, g, w, b):
     return out
 
 
+# block shapes of the 3x3 kernel: (rows per wave, 32-channel tiles per wave, waves along channels)
+SHAPES = [(1, 1, 1), (2, 1, 1), (4, 1, 1), (1, 2, 1), (2, 2, 1), (1, 1, 2), (2, 1, 2), (1, 2, 2), (2, 2, 2),
+          (1, 1, 4), (2, 1, 4), (1, 2, 4), (2, 2, 4)]
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("N,H,W,Ci,Co", [(1, 4, 32, 32, 32), (2, 8, 64, 64, 32), (1, 6, 32, 32, 128),
-                                          (2, 16, 32, 96, 64)])
+                                          (2, 16, 32, 96, 64), (1, 5, 64, 128, 256)])
 def test_mfma_conv_matches_oracle(N, H, W, Ci, Co):
     print(f"case: conv {N}x{H}x{W} {Ci}->{Co}", flush=True)
     rng = np.random.default_rng(N * 1000 + H + Ci)
@@ -83,15 +88,20 @@ def test_mfma_conv_matches_oracle(N, H, W, Ci, Co):
     w = cr.bf16(0.1 * rng.standard_normal((9, Co, Ci)))
     b = rng.standard_normal(Co).astype(np.float32)
     g = cr.bf16(rng.standard_normal((N, H, W, Co)))
-    # forward: bf16 inputs are exact, accumulation in f32
-    got = _run(0, N, H, W, Ci, Co, x, None, w, b).reshape(N, H, W, Co)
-    ref = cr.conv3x3(x, w, b)
+    ref_f = cr.conv3x3(x, w, b)
+    ref_d = cr.conv3x3_dgrad(g, w)
     scale = np.sqrt(9 * Ci) * 0.1
-    assert np.abs(got - ref).max() < 1e-5 * scale * 10
-    # input gradient
-    got = _run(1, N, H, W, Ci, Co, g, None, w, None).reshape(N, H, W, Ci)
-    ref = cr.conv3x3_dgrad(g, w)
-    assert np.abs(got - ref).max() < 1e-4 * np.sqrt(9 * Co)
+    for tile in range(len(SHAPES) + 1):
+        nw = 32 * SHAPES[tile - 1][1] * SHAPES[tile - 1][2] if tile else 32
+        for split in (0, 1, 2, 4):
+            # forward: bf16 inputs are exact, accumulation in f32
+            if Co % nw == 0:
+                got = _run(0 | tile << 4 | split << 8, N, H, W, Ci, Co, x, None, w, b).reshape(N, H, W, Co)
+                assert np.abs(got - ref_f).max() < 1e-5 * scale * 10, (tile, split)
+            # input gradient: the same kernel with Cin and Cout exchanged
+            if Ci % nw == 0:
+                got = _run(1 | tile << 4 | split << 8, N, H, W, Ci, Co, g, None, w, None).reshape(N, H, W, Ci)
+                assert np.abs(got - ref_d).max() < 1e-4 * np.sqrt(9 * Co), (tile, split)
     # weight gradient
     got = _run(2, N, H, W, Ci, Co, x, g, None, None).reshape(9, Co, Ci)
     ref = cr.conv3x3_wgrad(x, g)
@@ -108,8 +118,11 @@ def test_unet_reference_shapes_and_param_count():
 
 def _sample_inputs(B, H, W, seed):
     rng = np.random.default_rng(seed)
-    x = rng.standard_normal((B, H, W, 4)).astype(np.float32)
+    x = np.zeros((B, H, W, 8), np.float32)
+    x[..., 1:4] = rng.standard_normal((B, H, W, 3))
     x[..., 0] = (rng.random((B, H, W)) < 0.1).astype(np.float32)
+    x[..., 4] = rng.uniform(0.1, 1.5)  # (tau - 1/2) / 0.2
+    x[..., 5] = rng.uniform(0.2, 1.0)  # u_in / s_u
     t = rng.standard_normal((B, H, W, 3)).astype(np.float32) * 0.3
     w = (x[..., 0] == 0).astype(np.float32)
     return x, t, w
@@ -153,8 +166,8 @@ def test_unet_gradients_match_finite_differences():
     for ks, ci, co, _ in unet_ref.SPECS:
         nw = ks * ks * ci * co
         live = np.arange(offs, offs + nw)
-        if offs == 0:  # first layer: only the 4 real input channels carry gradient
-            live = live[(live - offs) % ci < 4]
+        if offs == 0:  # first layer: only the 6 input channels with data carry gradient
+            live = live[(live - offs) % ci < 6]
         picks += list(rng.choice(live, 2, replace=False)) + [offs + nw + int(rng.integers(co if offs else co))]
         offs += nw + co
     picks = [i for i in picks if abs(g[i]) > 1e-6]
@@ -231,4 +244,100 @@ def test_rollout_jump_path():
     assert st.jumps == 4 and st.fallbacks == 0 and a.get_compute_num() == b.get_compute_num()
     err = velocity_error(a.fields(), b.fields(), mask)
     print(f"untrained-network rollout error vs plain LBM: {err:.3e}; residuals {st.residuals}")
+    assert np.isfinite(err) and err < 1.0
+
+
+def test_pool_fields_brute_force():
+    from lbm_amd.surrogate import fields_to_input, pool_fields
+    rng = np.random.default_rng(4)
+    H, W, F = 6, 8, 2
+    mask = (rng.random((H, W)) < 0.4).astype(np.uint8)
+    mask[0:2, 0:2] = 1  # one all-solid pixel
+    rho, ux, uy = (rng.standard_normal((H, W)) for _ in range(3))
+    solid, nf, (r, u, v) = pool_fields(mask, rho, ux, uy, F)
+    for py in range(H // F):
+        for px in range(W // F):
+            cells = [(F * py + j, F * px + i) for j in range(F) for i in range(F)]
+            fl = [c for c in cells if mask[c] != 1]
+            assert solid[py, px] == 1 - len(fl) / F ** 2
+            for got, a in ((r, rho), (u, ux), (v, uy)):
+                want = np.mean([a[c] for c in fl]) if fl else 0.0
+                assert abs(got[py, px] - want) < 1e-12
+    x = fields_to_input(mask, rho, ux, uy, 0.6, 0.05, F)
+    assert x.shape == (H // F, W // F, 8)
+    assert np.allclose(x[..., 4], 0.5) and np.allclose(x[..., 5], 0.5) and not x[..., 6:].any()
+    assert x[0, 0, 1:4].tolist() == [0.0, 0.0, 0.0]
+
+
+def _feq(rho, ux, uy_mem):
+    cxs = np.array([-1, 0, 1, -1, 0, 1, -1, 0, 1], np.float64)
+    cys = np.array([-1, -1, -1, 0, 0, 0, 1, 1, 1], np.float64)
+    ws = np.array([1 / 36, 1 / 9, 1 / 36, 1 / 9, 4 / 9, 1 / 9, 1 / 36, 1 / 9, 1 / 36])
+    cu = cxs[:, None, None] * ux + cys[:, None, None] * uy_mem
+    return ws[:, None, None] * rho * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (ux ** 2 + uy_mem ** 2))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("factor", [1, 2])
+def test_jump_with_constant_prediction(factor):
+    """A network that outputs the constant change c: every streamed fluid cell
+    becomes feq(rho0 + s_rho c0, u0 + s_u (c1, c2)) of its own pre-jump state
+    (bilinear interpolation of a constant is that constant)."""
+    print(f"case: constant jump factor {factor}", flush=True)
+    from lbm_amd import LBM, gen_circles
+    from lbm_amd.surrogate import S_RHO, S_U, UNet
+    X, Y = 512, 64
+    mask = gen_circles(X, Y, 11, 3, 4, 8)
+    g = LBM(1 / 0.6, X, Y)
+    g.mask = mask
+    g.custom_speed(0.05)
+    g.step(50)
+    net = UNet(1, Y // factor, X // factor, seed=1)
+    p = np.zeros(net.num_params, np.float32)
+    c = np.array([0.7, -0.4, 0.3], np.float32)
+    p[-32:-29] = c
+    net.set_params(p)
+    f0 = g.read_f().astype(np.float64)
+    n0 = g.get_compute_num()
+    g.surrogate_jump(net, 64)
+    assert g.get_compute_num() == n0 + 64
+    f1 = g.read_f().astype(np.float64)
+    cxs = np.array([-1, 0, 1, -1, 0, 1, -1, 0, 1], np.float64)
+    cys = np.array([-1, -1, -1, 0, 0, 0, 1, 1, 1], np.float64)
+    rho0 = f0.sum(0)
+    ux0 = np.tensordot(cxs, f0, 1) / rho0
+    uym0 = np.tensordot(cys, f0, 1) / rho0  # memory-row direction (southwards)
+    want = _feq(rho0 + S_RHO * c[0], ux0 + S_U * c[1], uym0 - S_U * c[2])
+    streamed = (mask != 1)
+    streamed[:, 0] = False
+    streamed[-1, :] = False
+    err = np.abs(f1 - want)[:, streamed].max()
+    assert err < 2e-6, err
+    keep = ~streamed
+    assert np.array_equal(f1[:, keep], f0[:, keep])
+    g.close()
+    net.close()
+
+
+@pytest.mark.gpu
+def test_rollout_coarse_network_jump_path():
+    """A half-resolution network drives the rollout; bookkeeping and finiteness."""
+    from lbm_amd import LBM, gen_circles
+    from lbm_amd.rollout import Rollout, velocity_error
+    from lbm_amd.surrogate import UNet
+    X, Y = 512, 64
+    mask = gen_circles(X, Y, 3, 4, 3, 8)
+    net = UNet(1, Y // 2, X // 2, seed=2)
+
+    def make():
+        g = LBM(1 / 0.56, X, Y)
+        g.mask = mask
+        g.custom_speed(0.08)
+        g.step(200)
+        return g
+    a, b = make(), make()
+    st = Rollout(a, net, threshold=float("inf")).run(72 * 3)
+    b.step(72 * 3)
+    assert st.jumps == 3 and a.get_compute_num() == b.get_compute_num()
+    err = velocity_error(a.fields(), b.fields(), mask)
     assert np.isfinite(err) and err < 1.0
