@@ -244,7 +244,7 @@ const char *lbm_unet_last_error(void);
  * height x width (height % 8 == 0, width % 256 == 0).  Input per pixel (NHWC,
  * 8 f32 on the host): solid fraction, (rho - 1) / s_rho, ux / s_u, uy / s_u,
  * (tau - 1/2) / 0.2, u_in / s_u, 0, 0.  Output: the change of (rho, ux, uy) over
- * one jump in the same units.  Parameters are f32 masters (weights
+ * one jump in units d_rho, d_u.  Parameters are f32 masters (weights
  * [taps][cout][cin], then the bias, per layer). */
 typedef struct lbm_unet lbm_unet;
 int lbm_unet_create(int batch, int height, int width, uint64_t seed, int device, lbm_unet **out);
@@ -264,21 +264,24 @@ int lbm_unet_train_step(lbm_unet *net, const float *x, const float *target, cons
 /* device time of one forward pass (ms), averaged over reps */
 int lbm_unet_time_forward(lbm_unet *net, int reps, float *ms);
 
-/* Surrogate jump on a grid (same device): the network predicts the change of
- * rho and u over `steps` steps from the current state, in units s_rho, s_u;
- * every streamed fluid cell (solid cells, the inlet column and the last row keep
- * their populations) gets the equilibrium of its own rho, u plus that change,
- * and the step count advances by `steps`.  Net shape (1, ny, nx): one network
+/* Surrogate jump on a grid (same device): the network sees the current state
+ * with rho - 1 and u in units s_rho, s_u and predicts the change of rho and u
+ * over `steps` steps in units d_rho, d_u; every streamed fluid cell (solid
+ * cells, the inlet column and the last row keep their populations) gets the
+ * equilibrium of its own rho, u plus that change, and the step count advances
+ * by `steps`.  Net shape (1, ny, nx): one network
  * pixel per cell.  Net shape (1, ny / 2, nx / 2): each pixel sees a 2 x 2 block
  * (solid fraction, fluid means of rho and u) and the change is bilinearly
  * interpolated back to the cells. */
-int lbm_surrogate_jump(lbm_grid *g, lbm_unet *net, uint32_t steps, float s_rho, float s_u);
+int lbm_surrogate_jump(lbm_grid *g, lbm_unet *net, uint32_t steps, float s_rho, float s_u, float d_rho,
+                       float d_u);
 /* Device-side copy of the whole evolving state (one slot) and its restoration:
  * the rollout's fallback. */
 int lbm_snapshot(lbm_grid *g);
 int lbm_restore(lbm_grid *g);
 /* Velocity residual: mark the current velocity field, then later
- * rel = |u - u_mark| / |u_mark| over fluid cells (FP64 sums on the device). */
+ * rel = |u - u_mark| / |u_mark| over fluid cells (FP64 sums on the device);
+ * rel is +inf when either field holds a NaN or Inf. */
 int lbm_velocity_mark(lbm_grid *g);
 int lbm_velocity_change(lbm_grid *g, double *rel);
 

@@ -117,13 +117,13 @@ __global__ void k_pack_net_input(const float *__restrict__ mom, const uint8_t *_
     for (int c = 1; c < C / 8; ++c) out[c] = u16x8{};
 }
 
-// Streamed fluid cells := feq(rho + s_rho d0, u + s_u (d1, d2)) in set `buf`
+// Streamed fluid cells := feq(rho + d_rho o0, u + d_u (o1, o2)) in set `buf`
 // (S-form), where (rho, u) is the cell's own current state and d the network's
 // predicted change, bilinearly interpolated from the nx / F x ny / F output
 // grid at the cell centre (exactly the cell's own pixel for F = 1).
 template <class St, int F>
 __global__ void k_apply_jump(const float *__restrict__ net_out, int C, const float *__restrict__ mom,
-                             const uint8_t *__restrict__ mask, typename St::T *buf, Geom g, float s_rho, float s_u) {
+                             const uint8_t *__restrict__ mask, typename St::T *buf, Geom g, float d_rho, float d_u) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= g.X) return;
@@ -142,9 +142,9 @@ __global__ void k_apply_jump(const float *__restrict__ net_out, int C, const flo
         d[c] = (1.0f - fy) * ((1.0f - fx) * o00 + fx * o01) + fy * ((1.0f - fx) * o10 + fx * o11);
     }
     const float rho0 = mom[2 * g.pstride + i];
-    const float rho = rho0 + s_rho * d[0];
-    const float ux = mom[i] / rho0 + s_u * d[1];
-    const float uym = -(mom[g.pstride + i] / rho0 + s_u * d[2]);  // memory rows grow southwards
+    const float rho = rho0 + d_rho * d[0];
+    const float ux = mom[i] / rho0 + d_u * d[1];
+    const float uym = -(mom[g.pstride + i] / rho0 + d_u * d[2]);  // memory rows grow southwards
     const float uu = ux * ux + uym * uym;
     const size_t sown = (size_t)y * g.ys + x;
 #pragma unroll
@@ -873,13 +873,18 @@ int lbm_velocity_change(lbm_grid *g, double *rel) {
         double s[3];
         HIPCHK(hipMemcpyAsync(s, g->sums, sizeof s, hipMemcpyDeviceToHost, g->stream));
         HIPCHK(hipStreamSynchronize(g->stream));
-        *rel = s[1] > 0.0 ? std::sqrt(s[0] / s[1]) : 0.0;
+        // a non-finite state (a diverged prediction) must read as an infinite change
+        if (!std::isfinite(s[0]) || !std::isfinite(s[1])) *rel = INFINITY;
+        else if (s[1] > 0.0) *rel = std::sqrt(s[0] / s[1]);
+        else *rel = s[0] > 0.0 ? INFINITY : 0.0;
     });
 }
 
-int lbm_surrogate_jump(lbm_grid *g, lbm_unet *net, uint32_t steps, float s_rho, float s_u) {
+int lbm_surrogate_jump(lbm_grid *g, lbm_unet *net, uint32_t steps, float s_rho, float s_u, float d_rho,
+                       float d_u) {
     return guarded(g, [&] {
-        check(net != nullptr && s_rho > 0.0f && s_u > 0.0f, "bad network or scales");
+        check(net != nullptr && s_rho > 0.0f && s_u > 0.0f && d_rho > 0.0f && d_u > 0.0f,
+              "bad network or scales");
         int b, h, w;
         unet_shape(net, &b, &h, &w);
         const bool fine = b == 1 && h == (int)g->Y && w == (int)g->X;
@@ -901,7 +906,7 @@ int lbm_surrogate_jump(lbm_grid *g, lbm_unet *net, uint32_t steps, float s_rho, 
         const float *o = unet_output_tensor(net);
 #define APPLY(ST, TT, F)                                                                                          \
     hipLaunchKernelGGL((k_apply_jump<ST, F>), grid, dim3(256), 0, g->stream, o, 32, g->now, g->mask, (TT *)g->f[g->cur], \
-                       g->geom, s_rho, s_u)
+                       g->geom, d_rho, d_u)
         if (g->precision == LBM_PRECISION_F32) {
             if (fine) APPLY(StoreF32, float, 1);
             else APPLY(StoreF32, float, 2);

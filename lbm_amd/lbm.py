@@ -255,10 +255,12 @@ class LBM:
         self._chk(self._lib.lbm_velocity_change(self._h, ctypes.byref(r)))
         return r.value
 
-    def surrogate_jump(self, net, steps: int = 64, s_rho: float = 0.01, s_u: float = 0.1):
-        """Predict `steps` ahead with a trained UNet of shape (1, ny, nx) and
-        re-equilibrate the streamed fluid cells from the prediction."""
-        self._chk(self._lib.lbm_surrogate_jump(self._h, net._h, steps, s_rho, s_u))
+    def surrogate_jump(self, net, steps: int = 64, s_rho: float = 0.01, s_u: float = 0.1, d_rho: float = 0.01,
+                       d_u: float = 0.003):
+        """Predict the change `steps` ahead with a trained UNet of shape (1, ny, nx)
+        or (1, ny / 2, nx / 2) and re-equilibrate the streamed fluid cells from
+        the prediction; s_*: input units, d_*: output units."""
+        self._chk(self._lib.lbm_surrogate_jump(self._h, net._h, steps, s_rho, s_u, d_rho, d_u))
 
     def save_state(self) -> bytes:
         n = ctypes.c_size_t()

@@ -22,7 +22,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .surrogate import S_RHO, S_U
+from .surrogate import D_RHO, D_U, S_RHO, S_U
 
 
 @dataclass
@@ -36,10 +36,10 @@ class RolloutStats:
 
 class Rollout:
     def __init__(self, grid, net, jump: int = 64, correctors: int = 8, threshold: float = 0.05,
-                 s_rho: float = S_RHO, s_u: float = S_U):
+                 s_rho: float = S_RHO, s_u: float = S_U, d_rho: float = D_RHO, d_u: float = D_U):
         self.grid, self.net = grid, net
         self.jump, self.correctors, self.threshold = jump, correctors, threshold
-        self.s_rho, self.s_u = s_rho, s_u
+        self.scales = (s_rho, s_u, d_rho, d_u)
 
     def run(self, steps: int) -> RolloutStats:
         st = RolloutStats()
@@ -47,7 +47,7 @@ class Rollout:
         per = self.jump + self.correctors
         while steps - st.steps >= per:
             g.snapshot()
-            g.surrogate_jump(self.net, self.jump, self.s_rho, self.s_u)
+            g.surrogate_jump(self.net, self.jump, *self.scales)
             g.velocity_mark()
             g.step(self.correctors)
             r = g.velocity_change()

@@ -15,8 +15,10 @@ import numpy as np
 
 from . import _native as N
 
-S_RHO = 0.01   # density deviation scale
-S_U = 0.1      # velocity scale
+S_RHO = 0.01   # input unit of rho - 1
+S_U = 0.1      # input unit of the velocity
+D_RHO = 0.01   # output unit of the change of rho over one jump
+D_U = 0.003    # output unit of the change of the velocity over one jump
 NET_IN = 8     # host input channels per pixel
 
 
@@ -26,43 +28,55 @@ def _fp(a):
 
 def pool_fields(mask, rho, ux, uy, factor: int = 1):
     """(..., H, W) cell fields -> (..., H/F, W/F) pixels of F x F cells: solid
-    fraction and the fluid-cell means of rho, ux, uy (0 where a pixel has no
-    fluid), as the grid's surrogate input kernel computes them."""
+    fraction, fluid-cell count and the fluid-cell means of rho, ux, uy (0 where
+    a pixel has no fluid), as the grid's surrogate input kernel computes them.
+    Returns (solid, nfluid, [rho, ux, uy])."""
     fl = (np.asarray(mask) != 1)
     F = factor
 
     def blocks(a):
-        a = np.asarray(a, np.float64)
+        a = np.asarray(a, np.float32)
+        if F == 1:
+            return a
         sh = a.shape[:-2] + (a.shape[-2] // F, F, a.shape[-1] // F, F)
         return a.reshape(sh).sum(axis=(-3, -1))
     nf = blocks(fl)
-    safe = np.maximum(nf, 1.0)
-    means = [np.where(nf > 0, blocks(np.where(fl, v, 0.0)) / safe, 0.0) for v in (rho, ux, uy)]
+    inv = np.where(nf > 0, 1.0 / np.maximum(nf, 1.0), 0.0).astype(np.float32)
+    means = [blocks(np.where(fl, v, 0.0)) * inv for v in (rho, ux, uy)]
     return 1.0 - nf / (F * F), nf, means
 
 
-def fields_to_input(mask, rho, ux, uy, tau: float, u_in: float, factor: int = 1, s_rho: float = S_RHO,
-                    s_u: float = S_U) -> np.ndarray:
-    """(..., H, W) fields -> (..., H/F, W/F, NET_IN) network input."""
-    solid, nf, (r, u, v) = pool_fields(mask, rho, ux, uy, factor)
+def input_from_pooled(solid, nf, means, tau: float, u_in: float, s_rho: float = S_RHO,
+                      s_u: float = S_U) -> np.ndarray:
     has = nf > 0
     x = np.zeros(solid.shape + (NET_IN,), np.float32)
     x[..., 0] = solid
-    x[..., 1] = np.where(has, (r - 1.0) / s_rho, 0.0)
-    x[..., 2] = np.where(has, u / s_u, 0.0)
-    x[..., 3] = np.where(has, v / s_u, 0.0)
+    x[..., 1] = np.where(has, (means[0] - 1.0) / s_rho, 0.0)
+    x[..., 2] = means[1] / s_u
+    x[..., 3] = means[2] / s_u
     x[..., 4] = (tau - 0.5) / 0.2
     x[..., 5] = u_in / s_u
     return x
 
 
-def target_from(mask, f0, f1, factor: int = 1, s_rho: float = S_RHO, s_u: float = S_U) -> np.ndarray:
-    """Change of the pooled (rho, ux, uy) between two field triples, in network units."""
+def target_from_pooled(nf, m0, m1, d_rho: float = D_RHO, d_u: float = D_U) -> np.ndarray:
+    has = nf > 0
+    d = [np.where(has, (m1[i] - m0[i]) / s, 0.0) for i, s in enumerate((d_rho, d_u, d_u))]
+    return np.ascontiguousarray(np.stack(d, axis=-1), np.float32)
+
+
+def fields_to_input(mask, rho, ux, uy, tau: float, u_in: float, factor: int = 1, s_rho: float = S_RHO,
+                    s_u: float = S_U) -> np.ndarray:
+    """(..., H, W) fields -> (..., H/F, W/F, NET_IN) network input."""
+    solid, nf, means = pool_fields(mask, rho, ux, uy, factor)
+    return input_from_pooled(solid, nf, means, tau, u_in, s_rho, s_u)
+
+
+def target_from(mask, f0, f1, factor: int = 1, d_rho: float = D_RHO, d_u: float = D_U) -> np.ndarray:
+    """Change of the pooled (rho, ux, uy) between two field triples, in output units."""
     _, nf, m0 = pool_fields(mask, *f0, factor)
     _, _, m1 = pool_fields(mask, *f1, factor)
-    has = nf > 0
-    d = [np.where(has, (m1[i] - m0[i]) / s, 0.0) for i, s in enumerate((s_rho, s_u, s_u))]
-    return np.ascontiguousarray(np.stack(d, axis=-1), np.float32)
+    return target_from_pooled(nf, m0, m1, d_rho, d_u)
 
 
 def loss_weights(mask, factor: int = 1) -> np.ndarray:
