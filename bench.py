@@ -11,9 +11,10 @@ before the timed region; K steps run between two stream synchronisations.
 Printed: ONE JSON line with the FP32 MLUPS as `value`, the roofline of the
 fused step kernel (72 algorithmic bytes per cell update, average kernel
 duration from HIP events recorded on the grid's own stream), the same for
-FP16-shifted storage (36 B per cell update), and the CPU baseline: the oracle's
-faithful f32 restatement of the reference's eight passes, timed on a bounded
-sample of the same workload on the host cores.
+FP16-shifted storage (36 B per cell update), the surrogate rollout's speed-up
+and velocity error against plain FP32 LBM (surrogate_leg), and the CPU
+baseline: the oracle's faithful f32 restatement of the reference's eight
+passes, timed on a bounded sample of the same workload on the host cores.
 """
 from __future__ import annotations
 
@@ -69,6 +70,63 @@ def run_gpu(precision, X, Y, mask, steps, warmup, omega, u_in, variant=None):
     return wall, dev_ms, cons, tuned
 
 
+SURROGATE_WEIGHTS = os.path.join(ROOT, "lbm_amd", "data", "surrogate_f2.npy")
+
+
+def surrogate_leg(steps=10000):
+    """Surrogate + LBM-corrector rollout against plain FP32 LBM on 1024 x 512.
+
+    12 seeded circles (r 8..32, seed 2024), tau 0.6, u_in 0.05, 2000 plain
+    spin-up steps; then `steps` steps of plain LBM and of the rollout (jump 64
+    with the half-resolution network, 8 corrector steps, residual gate 0.05)
+    from the same state.  Reported: the wall-clock speed-up (both sides end
+    with a stream sync), the velocity rel-L2 of the rollout against the plain
+    LBM at the end and after 1 and 10 cycles, and the same short-horizon
+    errors for the network-free ablation (the change predicted as zero:
+    re-equilibration + correctors only)."""
+    import numpy as np
+
+    from lbm_amd import LBM, gen_circles
+    from lbm_amd.rollout import Rollout, compare, velocity_error
+    from lbm_amd.surrogate import UNet, load_params
+    if not os.path.exists(SURROGATE_WEIGHTS):
+        return {"skipped": f"no trained weights at {os.path.relpath(SURROGATE_WEIGHTS, ROOT)}"}
+    X, Y, tau, u = 1024, 512, 0.6, 0.05
+    mask = gen_circles(X, Y, SEED, 12, 8, 32)
+
+    def make():
+        g = LBM(1.0 / tau, X, Y)
+        g.mask = mask
+        g.custom_speed(u)
+        g.step(2000)
+        return g
+    net = UNet(1, Y // 2, X // 2, seed=1)
+    params = load_params(SURROGATE_WEIGHTS)
+    out = {"workload": f"{X}x{Y}, 12 circles r 8-32 (seed {SEED}), tau {tau}, u_in {u}, after 2000 steps; "
+                       f"jump 64 with a {X // 2}x{Y // 2} U-Net, 8 corrector steps, residual gate 0.05",
+           "steps": steps}
+    short = {}
+    for label, p in (("trained", params), ("zero_change", np.zeros_like(params))):
+        net.set_params(p)
+        errs = []
+        for cycles in (1, 10):
+            a, b = make(), make()
+            Rollout(a, net, threshold=float("inf")).run(72 * cycles)
+            b.step(72 * cycles)
+            errs.append(round(velocity_error(a.fields(), b.fields(), mask), 5))
+            a.close()
+            b.close()
+        short[label] = {"u_rel_l2_after_1_cycle": errs[0], "u_rel_l2_after_10_cycles": errs[1]}
+    net.set_params(params)
+    speedup, err, st, t_plain, t_roll = compare(make, net, steps, threshold=0.05)
+    fwd_ms = net.time_forward(10)
+    net.close()
+    out.update({"speedup": round(speedup, 3), "u_rel_l2_at_end": round(err, 5), "jumps": st.jumps,
+                "fallbacks": st.fallbacks, "plain_ms": round(t_plain * 1e3, 2), "rollout_ms": round(t_roll * 1e3, 2),
+                "unet_forward_ms": round(fwd_ms, 4), "short_horizon": short})
+    return out
+
+
 def cpu_baseline(X, Y, mask, omega, u_in, budget_s):
     import numpy as np
 
@@ -107,6 +165,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f16", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--variant", type=int, default=None, help="FP32 kernel variant (default: autotune)")
     ap.add_argument("--f16-variant", type=int, default=None, help="FP16-shifted kernel variant (default: autotune)")
     args = ap.parse_args()
@@ -186,6 +245,11 @@ def main():
             "fp64_mass_momentum": [float(c) for c in c16],
             "autotune": tuned16,
         }
+
+    if not args.no_surrogate:
+        log("surrogate rollout vs plain LBM (1024x512, 10k steps)")
+        out["surrogate"] = surrogate_leg()
+        log(f"surrogate: {out['surrogate']}")
 
     if not args.no_cpu_baseline:
         log(f"cpu baseline: ~{args.cpu_budget:.0f} s of refcpu")

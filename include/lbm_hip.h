@@ -233,6 +233,27 @@ int lbm_batch_autotune(lbm_batch *b, uint32_t steps, int *best, float *ms);
 int lbm_batch_read_f(lbm_batch *b, uint32_t domain, float *f9);
 int lbm_batch_read_moments(lbm_batch *b, uint32_t domain, float *jx, float *jy, float *rho);
 
+/* ---- FP64 solver (no reference counterpart) ------------------------------------------ */
+
+/* The textbook FP64 D2Q9 scheme (BGK collide of every cell, then a pull stream
+ * with half-way bounce-back) with the reference step's boundary rules, on the
+ * GPU: the yardstick for the error of the f32 and FP16-shifted grids at full
+ * size.  Starts at feq(1, (u_in, 0)) with walls on rows 0 and ny - 1. */
+typedef struct lbm_f64 lbm_f64;
+const char *lbm_f64_last_error(void);
+int lbm_f64_create(uint32_t nx, uint32_t ny, double omega, double u_in, int32_t boundary, int32_t device,
+                   lbm_f64 **out);
+int lbm_f64_destroy(lbm_f64 *h);
+/* one byte per cell, 1 = solid */
+int lbm_f64_set_mask(lbm_f64 *h, const uint8_t *solid);
+int lbm_f64_step(lbm_f64 *h, uint32_t n);
+int lbm_f64_sync(lbm_f64 *h);
+int lbm_f64_step_count(const lbm_f64 *h, uint64_t *n);
+/* rho, ux, uy (north-positive) of the current state, nx*ny doubles each (any may be NULL) */
+int lbm_f64_macro(lbm_f64 *h, double *rho, double *ux, double *uy);
+/* the current (pre-collision) state, 9 dense nx*ny planes */
+int lbm_f64_read_f(lbm_f64 *h, double *f9);
+
 /* ---- learned surrogate (no reference counterpart) --------------------------------- */
 
 /* Text of the last error of an lbm_unet_* / lbm_conv3x3_test call on this thread. */

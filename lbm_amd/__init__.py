@@ -4,9 +4,10 @@ with this package as the host-side mirror of the reference's ``LBM`` type."""
 from ._native import (BOUNDARY_REFERENCE, BOUNDARY_ZERO_GRADIENT, PRECISION_F16S,
                       PRECISION_F32, LbmError, load)
 from .batch import Batch, gen_batch, gen_shapes
+from .f64 import LBM64
 from .lbm import LBM, SummaryStat, gen_circles, points_to_pairs, splitmix64
 
 __all__ = [
-    "LBM", "Batch", "gen_batch", "gen_shapes", "SummaryStat", "LbmError", "gen_circles", "points_to_pairs", "splitmix64", "load",
+    "LBM", "LBM64", "Batch", "gen_batch", "gen_shapes", "SummaryStat", "LbmError", "gen_circles", "points_to_pairs", "splitmix64", "load",
     "PRECISION_F32", "PRECISION_F16S", "BOUNDARY_REFERENCE", "BOUNDARY_ZERO_GRADIENT",
 ]

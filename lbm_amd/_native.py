@@ -131,6 +131,15 @@ SIGNATURES = {
     "lbm_velocity_mark": (_i32, [_P]),
     "lbm_velocity_change": (_i32, [_P, _dp]),
     "lbm_conv3x3_test": (_i32, [_i32, _i32, _i32, _i32, _i32, _i32, _fp, _fp, _fp, _fp, _fp, _i32]),
+    "lbm_f64_last_error": (ctypes.c_char_p, []),
+    "lbm_f64_create": (_i32, [_u32, _u32, ctypes.c_double, ctypes.c_double, _i32, _i32, _PP]),
+    "lbm_f64_destroy": (_i32, [_P]),
+    "lbm_f64_set_mask": (_i32, [_P, _bp]),
+    "lbm_f64_step": (_i32, [_P, _u32]),
+    "lbm_f64_sync": (_i32, [_P]),
+    "lbm_f64_step_count": (_i32, [_P, _u64p]),
+    "lbm_f64_macro": (_i32, [_P, _dp, _dp, _dp]),
+    "lbm_f64_read_f": (_i32, [_P, _dp]),
     "lbm_splitmix64": (_u64, [_u64p]),
     "lbm_gen_circles": (_i32, [_bp, _u32, _u32, _u64, _u32, _u32, _u32]),
 }
@@ -166,6 +175,13 @@ def check(code: int, handle=None):
 def check_unet(code: int):
     if code != LBM_OK:
         msg = load().lbm_unet_last_error()
+        raise LbmError(code, msg.decode() if msg else "")
+    return code
+
+
+def check_f64(code: int):
+    if code != LBM_OK:
+        msg = load().lbm_f64_last_error()
         raise LbmError(code, msg.decode() if msg else "")
     return code
 

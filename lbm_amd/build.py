@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "liblbm_hip.so")
-SOURCES = ["lbm_grid.hip", "lbm_batch.hip", "unet.hip"]
+SOURCES = ["lbm_grid.hip", "lbm_batch.hip", "unet.hip", "lbm_f64.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

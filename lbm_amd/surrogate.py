@@ -85,6 +85,21 @@ def loss_weights(mask, factor: int = 1) -> np.ndarray:
     return (1.0 - solid).astype(np.float32)
 
 
+def save_params(path, params) -> None:
+    """Parameters as bfloat16 bit patterns (round to nearest even), .npy."""
+    a = np.ascontiguousarray(params, np.float32).view(np.uint32)
+    r = ((a + 0x7FFF + ((a >> 16) & 1)) >> 16).astype(np.uint16)
+    np.save(path, r)
+
+
+def load_params(path) -> np.ndarray:
+    """Parameters saved by save_params (or a plain float32 .npy)."""
+    a = np.load(path, allow_pickle=False)
+    if a.dtype == np.uint16:
+        return (a.astype(np.uint32) << 16).view(np.float32)
+    return a.astype(np.float32)
+
+
 class UNet:
     def __init__(self, batch: int, height: int, width: int, seed: int = 0, device: int = 0):
         self._lib = N.load()

@@ -21,7 +21,8 @@ import time
 import numpy as np
 
 from .batch import Batch, gen_batch, gen_shapes
-from .surrogate import D_RHO, D_U, S_RHO, S_U, UNet, input_from_pooled, loss_weights, pool_fields, target_from_pooled
+from .surrogate import (D_RHO, D_U, S_RHO, S_U, UNet, input_from_pooled, loss_weights, pool_fields, save_params,
+                        target_from_pooled)
 
 
 def velocity(jx, jy, rho):
@@ -132,15 +133,16 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--iters", type=int, default=1000)
     ap.add_argument("--minibatch", type=int, default=8)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--domains", type=int, default=16)
     ap.add_argument("--factor", type=int, default=1, choices=(1, 2))
     ap.add_argument("--size", type=int, default=0, help="domain side (default 256 x factor)")
-    ap.add_argument("--out", default="surrogate.npy")
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--out", default="surrogate.npy", help="parameters, bfloat16 bit patterns (.npy)")
     a = ap.parse_args()
     size = a.size or 256 * a.factor
-    net, losses = train(a.iters, a.minibatch, a.lr, a.domains, size, a.factor)
-    np.save(a.out, net.get_params())
+    net, losses = train(a.iters, a.minibatch, a.lr, a.domains, size, a.factor, seed=a.seed)
+    save_params(a.out, net.get_params())
     print(f"saved {a.out}; loss {losses[:10].mean():.4e} -> {losses[-10:].mean():.4e}; "
           f"S_RHO={S_RHO} S_U={S_U} D_RHO={D_RHO} D_U={D_U} factor={a.factor}")
 

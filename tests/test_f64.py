@@ -1,0 +1,66 @@
+"""The GPU FP64 solver against the CPU FP64 oracle (bit for bit), and the
+FP16-shifted / f32 grids' error against it."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _mask(X, Y, seed):
+    from lbm_amd import gen_circles
+    return gen_circles(X, Y, seed, 3, 4, 12)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_f64_solver_equals_oracle64(mode):
+    """Same scheme, same operation order, IEEE double without contraction:
+    the whole state must agree bit for bit after 300 steps."""
+    print(f"case: f64 vs oracle64 mode {mode}", flush=True)
+    from lbm_amd import LBM64
+    X, Y, omega, u = 200, 80, 1.0 / 0.56, 0.1
+    mask = _mask(X, Y, 5 + mode)
+    g = LBM64(omega, X, Y, u, boundary=mode)
+    g.set_mask(mask)
+    g.step(300)
+    o = oracle.Oracle64(X, Y, omega, u, mode)
+    o.set_solid(mask)
+    o.step(300)
+    assert g.get_compute_num() == 300
+    fg, fo = g.read_f(), o.state()
+    assert np.array_equal(fg.view(np.uint64), fo.view(np.uint64))
+    for a, b in zip(g.macro(), o.macro()):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    g.close()
+
+
+def test_f16s_and_f32_error_against_f64_on_gpu():
+    """Config 1 (400 x 160, one circle r 16 at (80, 80), tau 0.56, u 0.1) for
+    1000 steps: the FP16-shifted grid's velocity error against FP64 stays under
+    1e-2 and within 20x the f32 grid's (the f32 error is the floor)."""
+    from lbm_amd import LBM, LBM64, PRECISION_F16S, PRECISION_F32
+    X, Y, omega, u = 400, 160, 1.0 / 0.56, 0.1
+    yy, xx = np.mgrid[0:Y, 0:X]
+    mask = ((xx - 80) ** 2 + (yy - 80) ** 2 <= 16 * 16).astype(np.uint8)
+    mask[0, :] = 1
+    mask[-1, :] = 1
+    ref = LBM64(omega, X, Y, u)
+    ref.set_mask(mask)
+    ref.step(1000)
+    rho64, ux64, uy64 = ref.macro()
+    fl = mask != 1
+    errs = {}
+    for p in (PRECISION_F32, PRECISION_F16S):
+        g = LBM(omega, X, Y, precision=p)
+        g.mask = mask
+        g.custom_speed(u)
+        g.step(1000)
+        rho, ux, uy, _ = g.fields()
+        num = ((ux - ux64) ** 2 + (uy - uy64) ** 2)[fl].sum()
+        errs[p] = float(np.sqrt(num / (ux64 ** 2 + uy64 ** 2)[fl].sum()))
+        g.close()
+    print(f"velocity rel-L2 vs FP64 after 1000 steps: f32 {errs[PRECISION_F32]:.3e}, "
+          f"f16s {errs[PRECISION_F16S]:.3e}")
+    assert errs[PRECISION_F16S] < 1e-2
+    assert errs[PRECISION_F32] < 1e-4

@@ -341,3 +341,40 @@ def test_rollout_coarse_network_jump_path():
     assert st.jumps == 3 and a.get_compute_num() == b.get_compute_num()
     err = velocity_error(a.fields(), b.fields(), mask)
     assert np.isfinite(err) and err < 1.0
+
+
+@pytest.mark.gpu
+def test_trained_network_beats_the_network_free_method():
+    """Ablation: the committed trained weights (lbm_amd/data, written by
+    lbm_amd/train.py on the batched LBM) against the same rollout with the
+    change predicted as zero (re-equilibration + correctors only), one cycle
+    of jump 64 + 8 correctors on an unseen geometry: the network must at
+    least halve the velocity error."""
+    import os
+    from lbm_amd import LBM, gen_circles
+    from lbm_amd.rollout import Rollout, velocity_error
+    from lbm_amd.surrogate import UNet, load_params
+    path = os.path.join(os.path.dirname(__file__), "..", "lbm_amd", "data", "surrogate_f2.npy")
+    params = load_params(path)
+    X, Y = 1024, 256
+    mask = gen_circles(X, Y, 99, 8, 8, 24)
+    net = UNet(1, Y // 2, X // 2, seed=1)
+    errs = {}
+    for label, p in (("trained", params), ("zero", np.zeros_like(params))):
+        net.set_params(p)
+
+        def make():
+            g = LBM(1 / 0.65, X, Y)
+            g.mask = mask
+            g.custom_speed(0.06)
+            g.step(1500)
+            return g
+        a, b = make(), make()
+        Rollout(a, net, threshold=float("inf")).run(72)
+        b.step(72)
+        errs[label] = velocity_error(a.fields(), b.fields(), mask)
+        a.close()
+        b.close()
+    net.close()
+    print(f"one-cycle velocity error: trained {errs['trained']:.4e}, zero-change {errs['zero']:.4e}")
+    assert errs["trained"] < 0.5 * errs["zero"]
