@@ -49,13 +49,11 @@
 namespace {
 
 // The step kernel of one grid: every cell, CPT consecutive cells per thread.
-// P-form: vector pulls patched in place where the bounce-back byte says so (the
-// bytes are read only in segments that are not clear);
+// P-form: vector pulls patched in place where the bounce-back byte says so;
 // S-form (no pull): the cell's own values, CPT = 1.
 template <class St, int MODE, bool PFORM, int OP, int CPT, bool NT>
 __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, typename St::T *dst,
                                                  const uint8_t *__restrict__ flags,
-                                                 const uint8_t *__restrict__ segs,
                                                  const uint8_t *__restrict__ mask, Geom g,
                                                  float *__restrict__ mom, double *__restrict__ partials,
                                                  int blocks_per_row, int nblocks, int remap) {
@@ -66,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, type
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     if (x0 < g.X) {
         if constexpr (PFORM)
-            patched_tile<St, MODE, OP, CPT, NT>(src, dst, flags, segs, mask, g, y, x0, mom, s0, s1, s2);
+            patched_tile<St, MODE, OP, CPT, NT>(src, dst, flags, mask, g, y, x0, mom, s0, s1, s2);
         else
             interior_tile<St, MODE, false, OP, CPT, NT, true>(src, dst, mask, nullptr, g, y, x0, mom, s0, s1, s2);
     }
@@ -210,7 +208,7 @@ struct lbm_grid {
     void *f[2] = {nullptr, nullptr};
     int cur = 0;
     bool pform = false;
-    uint8_t *mask = nullptr, *flags = nullptr, *segs = nullptr;
+    uint8_t *mask = nullptr, *flags = nullptr;
     float *lag = nullptr;   // jx, jy, rho planes: the renderer's density bind group
     float *now = nullptr;   // moments of the current state (scratch)
     float *out = nullptr;   // summary output plane (persistent)
@@ -243,7 +241,7 @@ struct lbm_grid {
 
     void release() {
         (void)hipSetDevice(dev);
-        for (void *p : {f[0], f[1], (void *)mask, (void *)flags, (void *)segs,
+        for (void *p : {f[0], f[1], (void *)mask, (void *)flags,
                         (void *)lag, (void *)now, (void *)out, (void *)dense, (void *)partials, (void *)sums,
                         (void *)snap, (void *)umark, (void *)vpart})
             if (p) (void)hipFree(p);
@@ -251,7 +249,7 @@ struct lbm_grid {
         umark = nullptr;
         vpart = nullptr;
         f[0] = f[1] = nullptr;
-        mask = flags = segs = nullptr;
+        mask = flags = nullptr;
         lag = now = out = dense = nullptr;
         partials = sums = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
@@ -266,8 +264,8 @@ struct lbm_grid {
         using T = typename St::T;
         const Shape sh = shape(CPT);
         hipLaunchKernelGGL((k_cell<St, MODE, PFORM, OP, CPT, NT>), dim3(sh.nblocks), dim3(kBlock), 0, stream,
-                           (const T *)src, (T *)dst, flags, segs, mask, geom, mom, part, sh.blocks_per_row,
-                           sh.nblocks, remap);
+                           (const T *)src, (T *)dst, flags, mask, geom, mom, part, sh.blocks_per_row, sh.nblocks,
+                           remap);
         HIPCHK(hipGetLastError());
         last_partials = (size_t)sh.nblocks;
     }
@@ -327,10 +325,6 @@ struct lbm_grid {
         else
             hipLaunchKernelGGL(k_build_flags<LBM_BOUNDARY_ZERO_GRADIENT>, grid, dim3(256), 0, stream, mask, flags,
                                geom);
-        HIPCHK(hipGetLastError());
-        const int srow = geom.pitch / 64 + kSegSlack;
-        hipLaunchKernelGGL(k_build_segments, dim3((srow + 255) / 256, Y), dim3(256), 0, stream, flags, segs,
-                           (uint32_t *)nullptr, (uint32_t *)nullptr, geom);
         HIPCHK(hipGetLastError());
     }
 
@@ -541,7 +535,6 @@ int lbm_create(const lbm_config *cfg, lbm_grid **out) {
         g->f[1] = dalloc<char>(g->set_bytes());
         g->mask = dalloc<uint8_t>(pstride);
         g->flags = dalloc<uint8_t>(pstride);
-        g->segs = dalloc<uint8_t>((size_t)(pitch / 64 + kSegSlack) * cfg->ny);
         g->lag = dalloc<float>(3 * (size_t)pstride);
         g->out = dalloc<float>(pstride);
         HIPCHK(hipMemsetAsync(g->lag, 0, sizeof(float) * 3 * pstride, g->stream));
