@@ -11,7 +11,8 @@ before the timed region; K steps run between two stream synchronisations.
 Printed: ONE JSON line with the FP32 MLUPS as `value`, the roofline of the
 fused step kernel (72 algorithmic bytes per cell update, average kernel
 duration from HIP events recorded on the grid's own stream), the same for
-FP16-shifted storage (36 B per cell update), the surrogate rollout's speed-up
+FP16-shifted storage (36 B per cell update), the batched configuration's
+MLUPS (batched_leg), the surrogate rollout's speed-up
 and velocity error against plain FP32 LBM (surrogate_leg), and the CPU
 baseline: the oracle's faithful f32 restatement of the reference's eight
 passes, timed on a bounded sample of the same workload on the host cores.
@@ -127,6 +128,34 @@ def surrogate_leg(steps=10000):
     return out
 
 
+def batched_leg(n_domains=512, steps=200):
+    """The batched configuration: 512 independent domains (sizes 128x64 ..
+    1024x512 in multiples of 32, 1-12 circles/rectangles, tau in [0.52, 0.8],
+    u_in in [0.02, 0.1], all from seed 2024) stepped together; tile variant
+    autotuned, hipGraph replay on.  Whole-batch MLUPS per precision."""
+    from lbm_amd import PRECISION_F16S, PRECISION_F32, Batch, gen_batch, gen_shapes
+    doms = gen_batch(SEED, n_domains)
+    masks = [gen_shapes(nx, ny, 10_000 + i, 1, 12) for i, (nx, ny, _, _) in enumerate(doms)]
+    out = {"workload": f"{n_domains} domains from seed {SEED} (128x64..1024x512, 1-12 shapes, tau 0.52-0.8, "
+                       f"u_in 0.02-0.1), {steps} timed steps, hipGraph replay"}
+    for name, prec, bpc in (("f32", PRECISION_F32, 72), ("f16s", PRECISION_F16S, 36)):
+        b = Batch(doms, precision=prec)
+        for i, m in enumerate(masks):
+            b.set_mask(i, m)
+        b.step(4)
+        cells = b.info()["cells"]
+        best, _ = b.autotune(10)
+        b.set_graph(True)
+        b.step(20)
+        b.sync()
+        ms = b.step_timed(steps)
+        s = ms / 1e3
+        out[name] = {"mlups": round(cells * steps / s / 1e6, 1), "cells": cells, "variant": best,
+                     "achieved_gbs": round(bpc * cells * steps / s / 1e9, 1), "us_per_step": round(s / steps * 1e6, 2)}
+        b.close()
+    return out
+
+
 def cpu_baseline(X, Y, mask, omega, u_in, budget_s):
     import numpy as np
 
@@ -166,6 +195,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f16", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--variant", type=int, default=None, help="FP32 kernel variant (default: autotune)")
     ap.add_argument("--f16-variant", type=int, default=None, help="FP16-shifted kernel variant (default: autotune)")
     args = ap.parse_args()
@@ -245,6 +275,11 @@ def main():
             "fp64_mass_momentum": [float(c) for c in c16],
             "autotune": tuned16,
         }
+
+    if not args.no_batched:
+        log("batched: 512 domains")
+        out["batched"] = batched_leg()
+        log(f"batched: {out['batched']}")
 
     if not args.no_surrogate:
         log("surrogate rollout vs plain LBM (1024x512, 10k steps)")

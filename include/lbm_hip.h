@@ -284,6 +284,9 @@ int lbm_unet_train_step(lbm_unet *net, const float *x, const float *target, cons
                         float *loss);
 /* device time of one forward pass (ms), averaged over reps */
 int lbm_unet_time_forward(lbm_unet *net, int reps, float *ms);
+/* lbm_surrogate_jump replays the forward pass from a hipGraph captured on the
+ * grid's stream (default on); off: plain launches.  Results are identical. */
+int lbm_unet_set_graph(lbm_unet *net, int enable);
 
 /* Surrogate jump on a grid (same device): the network sees the current state
  * with rho - 1 and u in units s_rho, s_u and predicts the change of rho and u

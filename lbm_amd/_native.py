@@ -125,6 +125,7 @@ SIGNATURES = {
     "lbm_unet_loss_grad": (_i32, [_P, _fp, _fp, _fp, _fp, _fp]),
     "lbm_unet_train_step": (_i32, [_P, _fp, _fp, _fp, _f32, _fp]),
     "lbm_unet_time_forward": (_i32, [_P, _i32, _fp]),
+    "lbm_unet_set_graph": (_i32, [_P, _i32]),
     "lbm_surrogate_jump": (_i32, [_P, _P, _u32, _f32, _f32, _f32, _f32]),
     "lbm_snapshot": (_i32, [_P]),
     "lbm_restore": (_i32, [_P]),

@@ -817,6 +817,10 @@ struct lbm_unet {
     size_t ws_elems = 0;
     std::vector<void *> mem;
     int adam_t = 0;
+    // the forward pass as a hipGraph (captured once; a graph runs on any stream)
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
 
     void *alloc(size_t bytes) {
         void *p = dalloc<char>(bytes);
@@ -831,8 +835,15 @@ struct lbm_unet {
         t.a = values ? (uint16_t *)alloc(t.n() * 2) : nullptr;
         t.g = (uint16_t *)alloc(t.n() * 2);
     }
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+    }
     void release() {
         (void)hipSetDevice(dev);
+        drop_graph();
         for (void *p : mem) (void)hipFree(p);
         mem.clear();
         if (stream) (void)hipStreamDestroy(stream);
@@ -1103,11 +1114,32 @@ void unet_shape(const lbm_unet *n, int *b, int *h, int *w) {
     *h = n->H;
     *w = n->W;
 }
+// The forward pass on stream s.  Every buffer it touches is fixed for the
+// network's lifetime (weights are updated in place), so it is captured once as
+// a hipGraph (no memset nodes) and replayed: at surrogate sizes the ~30
+// launches cost as much as their work.
 void unet_forward_on(lbm_unet *n, hipStream_t s) {
     hipStream_t saved = n->stream;
     n->stream = s;
     try {
-        n->forward();
+        if (!n->use_graph) {
+            n->forward();
+        } else {
+            if (!n->graph_exec) {
+                HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                try {
+                    n->forward();
+                } catch (...) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(s, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                HIPCHK(hipStreamEndCapture(s, &n->graph));
+                HIPCHK(hipGraphInstantiate(&n->graph_exec, n->graph, nullptr, nullptr, 0));
+            }
+            HIPCHK(hipGraphLaunch(n->graph_exec, s));
+        }
     } catch (...) {
         n->stream = saved;
         throw;
@@ -1140,6 +1172,15 @@ int lbm_unet_create(int batch, int height, int width, uint64_t seed, int device,
             throw;
         }
         *out = n;
+    });
+}
+
+int lbm_unet_set_graph(lbm_unet *n, int enable) {
+    return uguard([&] {
+        ucheck(n != nullptr, "null network");
+        HIPCHK(hipSetDevice(n->dev));
+        n->use_graph = enable != 0;
+        if (!n->use_graph) n->drop_graph();
     });
 }
 

@@ -161,6 +161,10 @@ class UNet:
         N.check_unet(self._lib.lbm_unet_train_step(self._h, _fp(x), _fp(t), _fp(w), lr, ctypes.byref(loss)))
         return loss.value
 
+    def set_graph(self, enable: bool):
+        """Replay the forward pass of surrogate jumps from a hipGraph (default) or not."""
+        N.check_unet(self._lib.lbm_unet_set_graph(self._h, int(enable)))
+
     def time_forward(self, reps: int = 5) -> float:
         ms = ctypes.c_float()
         N.check_unet(self._lib.lbm_unet_time_forward(self._h, reps, ctypes.byref(ms)))

@@ -378,3 +378,30 @@ def test_trained_network_beats_the_network_free_method():
     net.close()
     print(f"one-cycle velocity error: trained {errs['trained']:.4e}, zero-change {errs['zero']:.4e}")
     assert errs["trained"] < 0.5 * errs["zero"]
+
+
+@pytest.mark.gpu
+def test_rollout_graph_replay_equals_plain_launches():
+    """The jump's forward pass replayed from a hipGraph over several cycles,
+    with eager LBM work in between, gives the same bits as plain launches."""
+    from lbm_amd import LBM, gen_circles
+    from lbm_amd.rollout import Rollout
+    from lbm_amd.surrogate import UNet, load_params
+    import os
+    X, Y = 512, 64
+    mask = gen_circles(X, Y, 8, 4, 3, 8)
+    params = load_params(os.path.join(os.path.dirname(__file__), "..", "lbm_amd", "data", "surrogate_f2.npy"))
+    out = []
+    for graph in (True, False):
+        net = UNet(1, Y // 2, X // 2, seed=1)
+        net.set_params(params)
+        net.set_graph(graph)
+        g = LBM(1 / 0.6, X, Y)
+        g.mask = mask
+        g.custom_speed(0.05)
+        g.step(300)
+        Rollout(g, net, threshold=float("inf")).run(72 * 4)
+        out.append(g.read_f())
+        g.close()
+        net.close()
+    assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
