@@ -65,9 +65,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_interior(const typename St::T 
         const int y = (int)(e.y >> 16);
         const int x0 = (int)(e.y & 0xFFFFu) * 64 + (int)lane * CPT;
         if (x0 < d.g.X)
-            interior_tile<St, MODE, true, OP, CPT, NT, true>(src + d.f_off, dst + d.f_off, mask + d.aux_off, nullptr,
-                                                            d.g, y, x0, mom ? mom + 3 * d.aux_off : nullptr, s0, s1,
-                                                            s2);
+            interior_tile<St, MODE, true, OP, CPT, NT>(src + d.f_off, dst + d.f_off, mask + d.aux_off, d.g, y, x0,
+                                                      mom ? mom + 3 * d.aux_off : nullptr, s0, s1, s2);
     }
     if (OP & OP_REDUCE) block_sums(s0, s1, s2, partials, blockIdx.x);
 }

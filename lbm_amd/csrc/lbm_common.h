@@ -258,15 +258,13 @@ __device__ __forceinline__ void block_sums(double s0, double s1, double s2, doub
 
 // One interior tile: CPT consecutive cells of row y starting at x0, each
 // population moved as one vector (the nine pulls of a tile are contiguous in
-// their upstream rows, at most one element off alignment).  P-form pulls; the
-// nine loads are issued before the caller's clear test (`clear`, read from
-// `segs` unless LISTED) with row and column clamped into the set, so they are
-// always in bounds and overlap the test's latency.  S-form (no pull): own cell.
-template <class St, int MODE, bool PFORM, int OP, int CPT, bool NT, bool LISTED>
+// their upstream rows, at most one element off alignment).  P-form pulls, for
+// tiles the caller knows to be clear (every cell streamed, no bounce-back, not
+// at either end of the row); S-form (no pull): the cells' own values.
+template <class St, int MODE, bool PFORM, int OP, int CPT, bool NT>
 __device__ __forceinline__ void interior_tile(const typename St::T *src, typename St::T *dst,
-                                              const uint8_t *__restrict__ mask, const uint8_t *__restrict__ segs,
-                                              const Geom &g, int y, int x0, float *__restrict__ mom, double &s0,
-                                              double &s1, double &s2) {
+                                              const uint8_t *__restrict__ mask, const Geom &g, int y, int x0,
+                                              float *__restrict__ mom, double &s0, double &s1, double &s2) {
     using T = typename St::T;
     using V = typename Vec<T, CPT>::a;
     using U = typename Vec<T, CPT>::u;
@@ -274,23 +272,21 @@ __device__ __forceinline__ void interior_tile(const typename St::T *src, typenam
     const long long ps = g.pstride, ks = g.ks;
     const size_t own0 = (size_t)y * g.pitch + x0;  // auxiliary planes
     const size_t sown0 = (size_t)y * g.ys + x0;    // state
-    bool go = true;
     V v[9];
     if (PFORM) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
+            // clamped into the set: in bounds whatever the caller's list holds
             const int ry = min(max(y - cy(k), 0), g.Y - 1);
             const int rx = min(max(x0 - cx(k), 0), g.pitch - CPT);
             const size_t idx = (size_t)k * ks + (size_t)ry * g.ys + rx;
             if (cx(k) == 0) v[k] = vload<V, NT>((const V *)(src + idx));
             else v[k] = (V)vload<U, NT>((const U *)(src + idx));
         }
-        if (!LISTED) go = segs[(size_t)y * (g.pitch / 64 + kSegSlack) + x0 / 64] == 1;
     } else {
 #pragma unroll
         for (int k = 0; k < 9; ++k) v[k] = vload<V, NT>((const V *)(src + (size_t)k * ks + sown0));
     }
-    if (!go) return;
     V o[9];
     VF mjx, mjy, mrho;
     uint64_t fluid_bits = ~0ull;

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, type
         if constexpr (PFORM)
             patched_tile<St, MODE, OP, CPT, NT>(src, dst, flags, mask, g, y, x0, mom, s0, s1, s2);
         else
-            interior_tile<St, MODE, false, OP, CPT, NT, true>(src, dst, mask, nullptr, g, y, x0, mom, s0, s1, s2);
+            interior_tile<St, MODE, false, OP, CPT, NT>(src, dst, mask, g, y, x0, mom, s0, s1, s2);
     }
     if (OP & OP_REDUCE) block_sums(s0, s1, s2, partials, blockIdx.x);
 }

@@ -55,60 +55,31 @@ struct ConvIn {
 
 typedef uint16_t u16x16 __attribute__((ext_vector_type(16)));  // 32 B: one lane's operands for two k-steps
 
-// out[p][co] = act(bias[co] + sum_{tap, ci} in[p + off(tap)][ci] w[tap][co][ci]),
-// w [taps][Cout][Cin] bf16.  Implicit GEMM, M = pixels, N = Cout, K = taps x Cin.
-// A wave computes MI runs of 32 pixels x NI tiles of 32 couts.  K advances in
-// chunks of 32 channels of one tap; lane half h loads channels 16h .. 16h + 15 of
-// the chunk as one 32-byte load, the first k-step consumes their first eight and
-// the second k-step their last eight (the same permutation of K for A and B).
-// Operands of the next chunk are loaded while the current one is multiplied.
-// SPLIT > 1: block z takes taps [z * 9 / SPLIT, (z + 1) * 9 / SPLIT) and writes
-// raw f32 sums to part[z][p][co]; k_conv_finish adds them up.
-// Requirements: W % 32 == 0, P % (32 * MI) == 0, ca % 32 == cb % 32 == 0,
+// 1x1 convolution: out[p][co] = act(bias[co] + sum_ci in[p][ci] w[co][ci]),
+// w [Cout][Cin] bf16, one plain input (ConvIn.a).  GEMM, M = pixels, N = Cout,
+// K = Cin.  A wave computes MI runs of 32 pixels x NI tiles of 32 couts.  K
+// advances in chunks of 32 channels; lane half h loads channels 16h .. 16h + 15
+// of the chunk as one 32-byte load, the first k-step consumes their first
+// eight and the second k-step their last eight (the same permutation of K for
+// A and B).  The next chunk's operands are loaded while the current one is
+// multiplied.  Requirements: P % (32 * MI) == 0, Cin % 32 == 0,
 // Cout % (32 * NI) == 0.  Only channels co < co_store are stored.
-template <int KS, int MI, int NI, int SPLIT, bool RELU, bool OUT_F32>
-__global__ __launch_bounds__(256) void k_conv(ConvIn src, const uint16_t *__restrict__ w,
-                                              const float *__restrict__ bias, void *__restrict__ out,
-                                              float *__restrict__ part, int N, int H, int W, int Cout, int co_store) {
-    constexpr int TAPS = KS * KS, R = KS / 2, TPS = TAPS / SPLIT;
-    const int Cin = src.ca + src.cb;
+template <int MI, int NI, bool RELU, bool OUT_F32>
+__global__ __launch_bounds__(256) void k_conv1x1(const uint16_t *__restrict__ in, const uint16_t *__restrict__ w,
+                                                 const float *__restrict__ bias, void *__restrict__ out, long long P,
+                                                 int Cin, int Cout, int co_store) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int half = lane >> 5, l32 = lane & 31;
-    const long long P = (long long)N * H * W;
     const long long p0 = ((long long)blockIdx.x * 4 + wave) * (32 * MI);
     if (p0 >= P) return;
     const int co0 = blockIdx.y * 32 * NI;
-    const int tap0 = blockIdx.z * TPS;
-    const int up = src.a_up ? 1 : 0;
-    const int Ha = H >> up, Wa = W >> up;
-    int ya[MI], xa[MI];
-    long long na[MI];
-#pragma unroll
-    for (int i = 0; i < MI; ++i) {
-        const long long p = p0 + 32 * i + l32;
-        xa[i] = (int)(p % W);
-        ya[i] = (int)((p / W) % H);
-        na[i] = p / ((long long)W * H);
-    }
-    const int nck = Cin >> 5;
-    const int nk = TPS * nck;
+    const int nk = Cin >> 5;
     auto load = [&](int kk, u16x16(&a)[MI], u16x16(&b)[NI]) {
-        const int tap = tap0 + kk / nck;
-        const int c0 = ((kk % nck) << 5) + 16 * half;
-        const int dy = tap / KS - R, dx = tap % KS - R;
+        const int c0 = (kk << 5) + 16 * half;
 #pragma unroll
-        for (int i = 0; i < MI; ++i) {
-            const int sy = ya[i] + dy, sx = xa[i] + dx;
-            const bool inside = sy >= 0 && sy < H && sx >= 0 && sx < W;
-            const int cy = inside ? sy : 0, cx = inside ? sx : 0;
-            const uint16_t *ptr = c0 < src.ca
-                ? src.a + ((na[i] * Ha + (cy >> up)) * Wa + (cx >> up)) * src.ca + c0
-                : src.b + ((na[i] * H + cy) * W + cx) * src.cb + (c0 - src.ca);
-            a[i] = inside ? *(const u16x16 *)ptr : u16x16{};
-        }
+        for (int i = 0; i < MI; ++i) a[i] = *(const u16x16 *)(in + (p0 + 32 * i + l32) * Cin + c0);
 #pragma unroll
-        for (int j = 0; j < NI; ++j)
-            b[j] = *(const u16x16 *)(w + ((size_t)tap * Cout + co0 + 32 * j + l32) * Cin + c0);
+        for (int j = 0; j < NI; ++j) b[j] = *(const u16x16 *)(w + (size_t)(co0 + 32 * j + l32) * Cin + c0);
     };
     f32x16 acc[MI][NI];
 #pragma unroll
@@ -147,19 +118,15 @@ __global__ __launch_bounds__(256) void k_conv(ConvIn src, const uint16_t *__rest
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
             const int co = co0 + 32 * j + l32;
-            const float bv = (SPLIT == 1 && bias) ? bias[co] : 0.0f;
+            const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const long long p = p0 + 32 * i + 8 * (r / 4) + 4 * half + (r % 4);
-                if (SPLIT > 1) {
-                    part[((long long)blockIdx.z * P + p) * Cout + co] = acc[i][j][r];
-                } else {
-                    float v = acc[i][j][r] + bv;
-                    if (RELU) v = fmaxf(v, 0.0f);
-                    if (co < co_store) {
-                        if (OUT_F32) ((float *)out)[p * Cout + co] = v;
-                        else ((uint16_t *)out)[p * Cout + co] = f2bf(v);
-                    }
+                float v = acc[i][j][r] + bv;
+                if (RELU) v = fmaxf(v, 0.0f);
+                if (co < co_store) {
+                    if (OUT_F32) ((float *)out)[p * Cout + co] = v;
+                    else ((uint16_t *)out)[p * Cout + co] = f2bf(v);
                 }
             }
         }
@@ -669,9 +636,9 @@ size_t conv_split_elems(int ks, long long P, int N, int H, int W, int Cin, int C
     return pl.split > 1 ? (size_t)pl.split * P * Cout : 0;
 }
 
-// Launch a convolution (also used for input gradients): 3x3 on the LDS-staged
-// kernel, 1x1 on the register-tiled one.  Only channels co < co_store are
-// stored (all of them when the K split is taken).
+// Launch a convolution (also used for input gradients): 3x3 on k_conv3_lds,
+// 1x1 on k_conv1x1.  Only channels co < co_store are stored (all of them when
+// the K split is taken).
 void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const float *bias, void *out, int N, int H,
               int W, int Cout, bool relu, bool out_f32, int co_store, float *ws, size_t ws_elems,
               ConvTile force = ConvTile{}) {
@@ -683,30 +650,23 @@ void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const 
     if (ks == 1) {
         ucheck(!in.a_up && in.cb == 0, "1x1 convolutions take one plain input");
         int mi = 1, ni = 1;
-        auto waves = [&](int m, int n_) { return P / (32 * m) * (Cout / (32 * n_)); };
-        auto ok = [&](int m, int n_) { return P % (32 * m) == 0 && Cout % (32 * n_) == 0; };
-        const int cand[2][2] = {{2, 1}, {1, 2}};
-        for (auto &c : cand)
-            if (ok(c[0], c[1]) && waves(c[0], c[1]) >= 4096) {
-                mi = c[0];
-                ni = c[1];
-                break;
-            }
+        if (P % 64 == 0 && P / 64 * (Cout / 32) >= 4096) mi = 2;
+        else if (Cout % 64 == 0 && P / 32 * (Cout / 64) >= 4096) ni = 2;
         const dim3 grid((unsigned)((P + 128 * mi - 1) / (128 * mi)), (unsigned)(Cout / (32 * ni)), 1);
-#define KCONV(MI, NI, R, F)                                                                                        \
-    hipLaunchKernelGGL((k_conv<1, MI, NI, 1, R, F>), grid, dim3(256), 0, s, in, w, bias, out, nullptr, N, H, W, \
-                       Cout, co_store)
-#define PICK(R, F)                                                         \
-    do {                                                                   \
-        if (mi == 2) KCONV(2, 1, R, F);                                    \
-        else if (ni == 2) KCONV(1, 2, R, F);                               \
-        else KCONV(1, 1, R, F);                                            \
+#define K1(MI, NI, R, F)                                                                                     \
+    hipLaunchKernelGGL((k_conv1x1<MI, NI, R, F>), grid, dim3(256), 0, s, in.a, w, bias, out, P, Cin, Cout, \
+                       co_store)
+#define PICK(R, F)                       \
+    do {                                 \
+        if (mi == 2) K1(2, 1, R, F);     \
+        else if (ni == 2) K1(1, 2, R, F); \
+        else K1(1, 1, R, F);             \
     } while (0)
         if (out_f32) PICK(false, true);
         else if (relu) PICK(true, false);
         else PICK(false, false);
 #undef PICK
-#undef KCONV
+#undef K1
         HIPCHK(hipGetLastError());
         return;
     }
