@@ -11,7 +11,8 @@ before the timed region; K steps run between two stream synchronisations.
 Printed: ONE JSON line with the FP32 MLUPS as `value`, the roofline of the
 fused step kernel (72 algorithmic bytes per cell update, average kernel
 duration from HIP events recorded on the grid's own stream), the same for
-FP16-shifted storage (36 B per cell update), the batched configuration's
+FP16-shifted storage (36 B per cell update), both again at 16384 x 8192
+(large_grid_leg), the batched configuration's
 MLUPS (batched_leg), the surrogate rollout's speed-up
 and velocity error against plain FP32 LBM (surrogate_leg), and the CPU
 baseline: the oracle's faithful f32 restatement of the reference's eight
@@ -107,17 +108,19 @@ def surrogate_leg(steps=10000):
                        f"jump 64 with a {X // 2}x{Y // 2} U-Net, 8 corrector steps, residual gate 0.05",
            "steps": steps}
     short = {}
+    horizons = (1, 2, 3, 5, 10)
     for label, p in (("trained", params), ("zero_change", np.zeros_like(params))):
         net.set_params(p)
-        errs = []
-        for cycles in (1, 10):
+        errs = {}
+        for cycles in horizons:
             a, b = make(), make()
             Rollout(a, net, threshold=float("inf")).run(72 * cycles)
             b.step(72 * cycles)
-            errs.append(round(velocity_error(a.fields(), b.fields(), mask), 5))
+            errs[cycles] = round(velocity_error(a.fields(), b.fields(), mask), 5)
             a.close()
             b.close()
-        short[label] = {"u_rel_l2_after_1_cycle": errs[0], "u_rel_l2_after_10_cycles": errs[1]}
+        short[label] = {f"u_rel_l2_after_{c}_cycles": e for c, e in errs.items()}
+        short[label]["cycles_within_1pct"] = max([c for c in horizons if errs[c] <= 0.01], default=0)
     net.set_params(params)
     speedup, err, st, t_plain, t_roll = compare(make, net, steps, threshold=0.05)
     fwd_ms = net.time_forward(10)
@@ -125,6 +128,23 @@ def surrogate_leg(steps=10000):
     out.update({"speedup": round(speedup, 3), "u_rel_l2_at_end": round(err, 5), "jumps": st.jumps,
                 "fallbacks": st.fallbacks, "plain_ms": round(t_plain * 1e3, 2), "rollout_ms": round(t_roll * 1e3, 2),
                 "unet_forward_ms": round(fwd_ms, 4), "short_horizon": short})
+    return out
+
+
+def large_grid_leg(steps=100):
+    """FP32 and FP16-shifted MLUPS at 16384 x 8192 (the large grid of the FP16
+    configuration), same obstacle generator and parameters as the main line,
+    variants autotuned."""
+    from lbm_amd import PRECISION_F16S, PRECISION_F32, gen_circles
+    X, Y = 16384, 8192
+    mask = gen_circles(X, Y, SEED, 200, 8, 64)
+    out = {"workload": f"{X}x{Y}, 200 circles r 8-64 (seed {SEED}), tau 0.53, u_in 0.08"}
+    for name, prec, bpc in (("f32", PRECISION_F32, 72), ("f16s", PRECISION_F16S, 36)):
+        wall, dev_ms, _, tuned = run_gpu(prec, X, Y, mask, steps, 10, 1.0 / 0.53, 0.08)
+        k = dev_ms / 1e3 / steps
+        out[name] = {"mlups": round(X * Y / k / 1e6, 1), "kernel_avg_us": round(k * 1e6, 2),
+                     "achieved_gbs": round(bpc * X * Y / k / 1e9, 1), "variant": tuned["variant"]}
+    out["f16s_speedup_vs_f32"] = round(out["f16s"]["mlups"] / out["f32"]["mlups"], 3)
     return out
 
 
@@ -196,6 +216,7 @@ def main():
     ap.add_argument("--no-f16", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--variant", type=int, default=None, help="FP32 kernel variant (default: autotune)")
     ap.add_argument("--f16-variant", type=int, default=None, help="FP16-shifted kernel variant (default: autotune)")
     args = ap.parse_args()
@@ -275,6 +296,11 @@ def main():
             "fp64_mass_momentum": [float(c) for c in c16],
             "autotune": tuned16,
         }
+
+    if not args.no_large and not args.no_f16:
+        log("16384x8192: f32 and f16s")
+        out["large_grid"] = large_grid_leg()
+        log(f"large grid: {out['large_grid']}")
 
     if not args.no_batched:
         log("batched: 512 domains")

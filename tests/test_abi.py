@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -127,3 +128,11 @@ def test_gen_circles_solid_fraction_config2(lib):
         m = gen_circles(nx, ny, 2024, 200, 8, 64, walls=False)
         frac = m.mean()
         assert lo < frac < hi, frac
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asked for more it exits with status 2 and says why."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr

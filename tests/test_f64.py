@@ -64,3 +64,14 @@ def test_f16s_and_f32_error_against_f64_on_gpu():
           f"f16s {errs[PRECISION_F16S]:.3e}")
     assert errs[PRECISION_F16S] < 1e-2
     assert errs[PRECISION_F32] < 1e-4
+
+
+def test_accuracy_driver_small_grid():
+    """The accuracy study end to end on a small grid: finite, f32 closer to FP64 than FP16-shifted."""
+    from lbm_amd import accuracy
+    lines = accuracy.run(256, 128, [200, 400], tau=0.6, u_in=0.05, circles=6, r_min=4, r_max=12)
+    assert [ln["step"] for ln in lines] == [200, 400]
+    for ln in lines:
+        assert ln["fp64_finite"] and ln["f32"]["finite"] and ln["f16s"]["finite"]
+        assert ln["f32"]["u_rel_l2"] < ln["f16s"]["u_rel_l2"] < 1e-2
+        assert abs(ln["f32"]["mass_rel_dev"]) < 1e-4
