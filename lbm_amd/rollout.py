@@ -12,8 +12,8 @@ One cycle advances the simulated time by ``jump + correctors`` steps:
    whole ``jump + correctors`` steps with the plain LBM instead (fallback).
 
 Time left over at the end (fewer than ``jump + correctors`` steps) runs plain.
-The plain LBM is the judge: ``compare`` runs both from the same state and
-reports the velocity error and the wall-clock speed-up.
+The plain LBM is the reference for the error: ``compare`` runs both from the
+same state and reports the velocity error and the wall-clock speed-up.
 """
 from __future__ import annotations
 

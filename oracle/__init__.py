@@ -1,7 +1,7 @@
 """CPU oracle for the D2Q9 simulation step — TEST INFRASTRUCTURE ONLY.
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg import
-this package, and only as the checker (or as the timed CPU baseline).  The
+this package, and only to check results (or as the timed CPU baseline).  The
 product package ``lbm_amd`` never imports it.
 
 Contents

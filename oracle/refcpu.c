@@ -1,7 +1,7 @@
 /*
  * refcpu — TEST INFRASTRUCTURE ONLY (the oracle).  Nothing in the product links,
  * imports or calls this file; tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg use it as the checker and as the timed CPU baseline.
+ * cpu_baseline leg use it to check results and as the timed CPU baseline.
  *
  * A faithful single-precision restatement of the reference lbm-wgpu simulation
  * step: the same 2 x 9 f32 structure-of-arrays planes, the same ping-pong parity,
