@@ -91,11 +91,13 @@ def _small_batch():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", [0, 1])
-def test_batch_domains_equal_single_grids(precision):
-    print(f"case: batch vs single grids precision {precision}", flush=True)
+@pytest.mark.parametrize("variant", [0, 3, 5])
+def test_batch_domains_equal_single_grids(precision, variant):
+    print(f"case: batch vs single grids precision {precision} variant {variant}", flush=True)
     from lbm_amd import LBM, Batch
     doms, masks = _small_batch()
     b = Batch(doms, precision=precision)
+    b.variant = variant
     for i, m in enumerate(masks):
         b.set_mask(i, m)
     b.step(1)
