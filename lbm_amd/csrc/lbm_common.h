@@ -339,9 +339,10 @@ __device__ __forceinline__ void interior_tile(const typename St::T *src, typenam
 //   bit set  -> the cell's own opposite population (half-way bounce-back);
 //   0xFF     -> not streamed: the other set's own values (or, if streamed with
 //               all eight upstream cells solid, every population bounces back).
-// The first and last group of a row (inlet column, outlet column with its
-// west-moving wrap / clamp, and the clamped vector loads) are gathered cell by
-// cell.  The rest of the work is shared by all cells.
+// The first and last group of a row are gathered cell by cell (one cell per
+// thread) or fix their clamped vector loads with a register shift and load the
+// outlet cell's wrapped / clamped upstream (several).  The rest of the work is
+// shared by all cells.
 template <class St, int MODE, int OP, int CPT, bool NT>
 __device__ __forceinline__ void patched_tile(const typename St::T *src, typename St::T *dst,
                                              const uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask,
@@ -371,17 +372,53 @@ __device__ __forceinline__ void patched_tile(const typename St::T *src, typename
     else if constexpr (CPT == 4) fw = *(const uint32_t *)(flags + own0);
     else if constexpr (CPT == 2) fw = *(const uint16_t *)(flags + own0);
     else fw = flags[own0];
-    // groups at either end of a row: the clamped vector loads above may be
-    // shifted by one element there, so every cell is gathered one by one
+    // Groups at either end of a row.  One cell per thread: the edge cell is
+    // gathered on its own (measured faster for CPT = 1).  Several cells per
+    // thread (measured faster for CPT > 1):
+    // the first group's east-moving loads were clamped from x0 - 1 = -1 to 0,
+    // one element too far; shift them back (element 0 is column 0, which is
+    // never streamed: the flag patch below rewrites it).
+    constexpr bool kGatherEdges = CPT == 1;
     const bool edge = x0 == 0 || x0 + CPT >= g.X;
-    if (fw != 0 || edge) {
+    if (!kGatherEdges && x0 == 0) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            if (cx(k) != 1) continue;
+#pragma unroll
+            for (int j = CPT - 1; j > 0; --j) set_lane<CPT>(v[k], j, get_lane<CPT>(v[k], j - 1));
+        }
+    }
+    // The last group's west-moving loads are one element short when clamped at
+    // the end of the row, and the outlet cell's upstream is the wrap / clamp of
+    // MODE (loaded in bounds; a cell whose upstream lies outside the grid is
+    // bounced back by the flag patch below).
+    if (!kGatherEdges && x0 + CPT >= g.X) {
+        const bool clamped = x0 + CPT == g.pitch;
+        const int jo = g.X - 1 - x0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            if (cx(k) != -1) continue;
+            if (clamped) {
+#pragma unroll
+                for (int j = 0; j + 1 < CPT; ++j) set_lane<CPT>(v[k], j, get_lane<CPT>(v[k], j + 1));
+            }
+            int ux, uy;
+            upstream<MODE>(k, g.X - 1, y, g.X, ux, uy);
+            ux = min(max(ux, 0), g.X - 1);
+            uy = min(max(uy, 0), g.Y - 1);
+#pragma unroll
+            for (int j = 0; j < CPT; ++j)
+                if (j == jo) set_lane<CPT>(v[k], j, src[(size_t)k * ks + (size_t)uy * g.ys + ux]);
+        }
+    }
+    if (fw != 0 || (kGatherEdges && edge)) {
 #pragma unroll
         for (int j = 0; j < CPT; ++j) {
             const int x = x0 + j;
             if (x >= g.X) break;
             const uint32_t fl = (uint32_t)(fw >> (8 * j)) & 0xFFu;
             const size_t sown = sown0 + j;
-            if (edge) {
+            if (kGatherEdges && edge) {
                 float f[9];
                 gather_cell<St, MODE, true>(src, dst, flags, mask, g, x, y, own0 + j, sown, f);
 #pragma unroll
