@@ -90,7 +90,11 @@ class PairSource:
 
 
 def train(iters=1000, minibatch=8, lr=1e-3, n_domains=16, size=256, factor=1, ring=256, seed=7, device=0,
-          log_every=100, net=None, verbose=True, tau=None, u_in=None, advance_every=4, refresh=40, spinup=300):
+          log_every=100, net=None, verbose=True, tau=None, u_in=None, advance_every=4, refresh=40, spinup=300,
+          time_budget=None):
+    """Train for `iters` iterations or until `time_budget` seconds have passed,
+    whichever comes first; the cosine learning-rate decay follows whichever
+    of the two is further along."""
     def source(k):
         return PairSource(n_domains=n_domains, size=size, factor=factor, seed=seed + 101 * k, device=device,
                           tau=tau, u_in=u_in, spinup=spinup)
@@ -103,6 +107,10 @@ def train(iters=1000, minibatch=8, lr=1e-3, n_domains=16, size=256, factor=1, ri
     losses = []
     t0 = time.perf_counter()
     for it in range(iters):
+        elapsed = time.perf_counter() - t0
+        if time_budget is not None and elapsed >= time_budget:
+            break
+        progress = max(it / max(1, iters), elapsed / time_budget if time_budget else 0.0)
         if it % advance_every == 0 or len(pool) < minibatch:
             if advances == refresh:
                 dropped += len(src.dropped)
@@ -116,11 +124,11 @@ def train(iters=1000, minibatch=8, lr=1e-3, n_domains=16, size=256, factor=1, ri
         x = np.stack([pool[j][0] for j in pick])
         y = np.stack([pool[j][1] for j in pick])
         w = np.stack([pool[j][2] for j in pick])
-        cur_lr = lr * 0.5 * (1 + np.cos(np.pi * it / max(1, iters)))  # cosine decay
+        cur_lr = lr * 0.5 * (1 + np.cos(np.pi * min(1.0, progress)))  # cosine decay
         losses.append(net.train_step(x, y, w, float(cur_lr)))
         if not np.isfinite(losses[-1]):
             raise FloatingPointError(f"non-finite training loss at iteration {it}")
-        if verbose and (it % log_every == 0 or it == iters - 1):
+        if verbose and it % log_every == 0:
             print(f"[train] iter {it} loss {losses[-1]:.4e} ({time.perf_counter() - t0:.1f} s)", flush=True)
     dropped += len(src.dropped)
     if verbose:
@@ -138,10 +146,12 @@ def main():
     ap.add_argument("--factor", type=int, default=1, choices=(1, 2))
     ap.add_argument("--size", type=int, default=0, help="domain side (default 256 x factor)")
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--time-budget", type=float, default=None, help="stop after this many seconds")
     ap.add_argument("--out", default="surrogate.npy", help="parameters, bfloat16 bit patterns (.npy)")
     a = ap.parse_args()
     size = a.size or 256 * a.factor
-    net, losses = train(a.iters, a.minibatch, a.lr, a.domains, size, a.factor, seed=a.seed)
+    net, losses = train(a.iters, a.minibatch, a.lr, a.domains, size, a.factor, seed=a.seed,
+                        time_budget=a.time_budget)
     save_params(a.out, net.get_params())
     print(f"saved {a.out}; loss {losses[:10].mean():.4e} -> {losses[-10:].mean():.4e}; "
           f"S_RHO={S_RHO} S_U={S_U} D_RHO={D_RHO} D_U={D_U} factor={a.factor}")
