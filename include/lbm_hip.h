@@ -56,6 +56,11 @@ extern "C" {
 #define LBM_SUMMARY_RHO 3
 #define LBM_SUMMARY_SPEED 4
 
+/* colour maps, in the order of the reference's ColorMap enum (lbm.rs:20-24) */
+#define LBM_COLORMAP_INFERNO 0
+#define LBM_COLORMAP_VIRIDIS 1
+#define LBM_COLORMAP_JET 2
+
 typedef struct lbm_grid lbm_grid;
 
 typedef struct lbm_config {
@@ -169,6 +174,12 @@ int lbm_read_fields(lbm_grid *g, float *rho, float *ux, float *uy, float *curl, 
  * persistent output plane that, like the reference's, keeps values the pass
  * skips; kind is LBM_SUMMARY_*. */
 int lbm_summary(lbm_grid *g, int kind, float *out);
+/* Replaces calculate_summary + color_map (lbm.rs:1051-1059, lbm.rs:1299-1305;
+ * color_map/{inferno,viridis,jet}.wgsl), fused in one pass: the summary plane
+ * of `kind` (kept as lbm_summary keeps it) mapped to colours, rgb = nx*ny*3
+ * floats (R, G, B per cell, rows of nx cells, row 0 first), solid cells black.
+ * The reference renders the same colours one quad per cell. */
+int lbm_render(lbm_grid *g, int kind, int colormap, float *rgb);
 
 /* FP64 sums of rho, jx, jy over the fluid cells (mask != 1) of the current
  * state.  Returns LBM_ENONFINITE if any sum is NaN or Inf. */

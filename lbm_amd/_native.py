@@ -91,6 +91,7 @@ SIGNATURES = {
     "lbm_read_moments": (_i32, [_P, _fp, _fp, _fp, _i32]),
     "lbm_read_fields": (_i32, [_P, _fp, _fp, _fp, _fp, _i32]),
     "lbm_summary": (_i32, [_P, _i32, _fp]),
+    "lbm_render": (_i32, [_P, _i32, _i32, _fp]),
     "lbm_conservation": (_i32, [_P, _dp]),
     "lbm_read_f": (_i32, [_P, _fp]),
     "lbm_write_f": (_i32, [_P, _fp]),

@@ -526,8 +526,47 @@ __global__ void k_walls(uint8_t *__restrict__ mask, Geom g) {
     mask[(size_t)(g.Y - 1) * g.pitch + x] = 1;
 }
 
-// summary passes (summary_stats/{curl,ux,uy,rho,speed}.wgsl); out keeps skipped cells
-__global__ void k_summary(const float *__restrict__ mom, float *__restrict__ out, Geom g, int kind) {
+// Colour maps (color_map/{inferno,viridis,jet}.wgsl): v is scaled and clamped
+// to [-n, n], its integer block picks two neighbouring control colours and the
+// fraction within the block blends them; solid cells are black.  The control
+// colours are the reference's, block -n first.
+__device__ __forceinline__ void colormap_rgb(int map, float v, bool solid, float *rgb) {
+    constexpr float kInferno[5][3] = {{0.98828125f, 1.0f, 0.64453125f},
+                                      {0.97265625f, 0.55859375f, 0.0390625f},
+                                      {0.73828125f, 0.21875f, 0.33203125f},
+                                      {0.34375f, 0.06640625f, 0.43359375f},
+                                      {0.0f, 0.0f, 0.01853125f}};
+    constexpr float kViridis[5][3] = {{0.9921875f, 0.90625f, 0.1484375f},
+                                      {0.3671875f, 0.7890625f, 0.3828125f},
+                                      {0.1328125f, 0.56640625f, 0.55078125f},
+                                      {0.23046875f, 0.32421875f, 0.546875f},
+                                      {0.265625f, 0.0078125f, 0.33203125f}};
+    constexpr float kJet[9][3] = {{0.0f, 0.0f, 0.5f}, {0.0f, 0.0f, 1.0f}, {0.0f, 0.5f, 1.0f},
+                                  {0.0f, 1.0f, 1.0f}, {0.5f, 1.0f, 0.5f}, {1.0f, 1.0f, 0.0f},
+                                  {1.0f, 0.5f, 0.0f}, {1.0f, 0.0f, 0.0f}, {0.5f, 0.0f, 0.0f}};
+    const bool jet = map == LBM_COLORMAP_JET;
+    const int n = jet ? 4 : 2;
+    const float c = fminf(fmaxf((jet ? 20.0f : 15.0f) * v, -(float)n), (float)n);
+    const int b = (int)floorf(c);
+    const float (*tab)[3] = jet ? kJet : (map == LBM_COLORMAP_VIRIDIS ? kViridis : kInferno);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (solid) {
+            rgb[k] = 0.0f;
+        } else if (b >= n) {
+            rgb[k] = tab[2 * n][k];
+        } else {
+            const float w = (float)(-b) + c;  // right_weight = -block + color
+            rgb[k] = (1.0f - w) * tab[b + n][k] + w * tab[b + n + 1][k];
+        }
+    }
+}
+
+// summary passes (summary_stats/{curl,ux,uy,rho,speed}.wgsl); out keeps the
+// values of cells a pass skips.  With rgb, the colour map `map` of the whole
+// plane follows in the same pass (3 floats per cell, dense rows).
+__global__ void k_summary(const float *__restrict__ mom, float *__restrict__ out, const uint8_t *__restrict__ mask,
+                          float *__restrict__ rgb, Geom g, int kind, int map) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= g.X) return;
@@ -537,7 +576,10 @@ __global__ void k_summary(const float *__restrict__ mom, float *__restrict__ out
     float v;
     switch (kind) {
     case LBM_SUMMARY_CURL: {
-        if (x == 0 || y >= g.Y - 1) return;
+        if (x == 0 || y >= g.Y - 1) {
+            v = out[i];  // not written by the curl pass: the previous value stays
+            break;
+        }
         // flattened neighbours: x+1 at the last column is column 0 of the next row
         const int xe = x + 1 < g.X ? x + 1 : 0, ye = x + 1 < g.X ? y : y + 1;
         const float uye = jy[(size_t)ye * g.pitch + xe];
@@ -553,6 +595,7 @@ __global__ void k_summary(const float *__restrict__ mom, float *__restrict__ out
     default: v = fminf(fmaxf(5.0f * sqrtf(jx[i] * jx[i] + jy[i] * jy[i]), 0.0f), 1.0f) - 0.5f; break;
     }
     out[i] = v;
+    if (rgb) colormap_rgb(map, v, mask[i] == 1, rgb + 3 * ((size_t)y * g.X + x));
 }
 
 // rho, u = j / rho, curl into four dense planes

@@ -213,6 +213,8 @@ struct lbm_grid {
     float *now = nullptr;   // moments of the current state (scratch)
     float *out = nullptr;   // summary output plane (persistent)
     float *dense = nullptr; // dense f32 staging (4 planes)
+    float *rgb = nullptr;   // colour-map output (3 floats per cell), allocated on first use
+    size_t rgb_elems = 0;
     double *partials = nullptr;
     double *sums = nullptr;
     size_t npartials = 0;
@@ -963,10 +965,30 @@ int lbm_read_fields(lbm_grid *g, float *rho, float *ux, float *uy, float *curl, 
 int lbm_summary(lbm_grid *g, int kind, float *out) {
     return guarded(g, [&] {
         check(kind >= 0 && kind <= 4, "unknown summary kind");
-        hipLaunchKernelGGL(k_summary, dim3((g->X + 255) / 256, g->Y), dim3(256), 0, g->stream, g->lag, g->out, g->geom,
-                           kind);
+        hipLaunchKernelGGL(k_summary, dim3((g->X + 255) / 256, g->Y), dim3(256), 0, g->stream, g->lag, g->out, g->mask,
+                           (float *)nullptr, g->geom, kind, 0);
         HIPCHK(hipGetLastError());
         if (out) g->copy_f32_plane_out(g->out, out);
+        HIPCHK(hipStreamSynchronize(g->stream));
+    });
+}
+
+int lbm_render(lbm_grid *g, int kind, int colormap, float *rgb) {
+    return guarded(g, [&] {
+        check(kind >= 0 && kind <= 4, "unknown summary kind");
+        check(colormap >= LBM_COLORMAP_INFERNO && colormap <= LBM_COLORMAP_JET, "unknown colour map");
+        check(rgb != nullptr, "null output");
+        const size_t n = (size_t)g->X * g->Y * 3;
+        if (g->rgb_elems < n) {
+            if (g->rgb) (void)hipFree(g->rgb);
+            g->rgb = nullptr;
+            g->rgb = dalloc<float>(n);
+            g->rgb_elems = n;
+        }
+        hipLaunchKernelGGL(k_summary, dim3((g->X + 255) / 256, g->Y), dim3(256), 0, g->stream, g->lag, g->out, g->mask,
+                           g->rgb, g->geom, kind, colormap);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(rgb, g->rgb, n * sizeof(float), hipMemcpyDeviceToHost, g->stream));
         HIPCHK(hipStreamSynchronize(g->stream));
     });
 }

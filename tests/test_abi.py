@@ -136,3 +136,15 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_save_ppm_writes_a_binary_ppm(tmp_path):
+    from lbm_amd import save_ppm
+    rgb = np.zeros((2, 3, 3), np.float32)
+    rgb[0, 0] = (1.0, 0.5, 0.0)
+    path = tmp_path / "f.ppm"
+    save_ppm(path, rgb)
+    data = path.read_bytes()
+    assert data.startswith(b"P6 3 2 255\n")
+    body = data[len(b"P6 3 2 255\n"):]
+    assert len(body) == 18 and body[:3] == bytes([255, 128, 0])

@@ -284,3 +284,26 @@ def test_refcpu_vs_oracle64_config1(oracle_lib):
     e_r = np.linalg.norm((rr - rho)[fl]) / np.linalg.norm((rho - 1)[fl])
     assert e_u < 1e-4, e_u
     assert e_r < 3e-3, e_r  # rel-L2 of (rho - 1): the denominator is ~1e-3 of rho
+
+
+# ---- colour maps (oracle/colormap_ref.py) ----------------------------------------
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_colormap_is_continuous_and_clamped(which):
+    """Piecewise-linear between control colours: continuous at every block
+    boundary (a flipped weight or a shifted table fails), constant beyond the
+    clamp (a wrong scale moves where that starts), black for solid cells."""
+    from oracle import colormap_ref as cm
+    s, n = (20.0, 4) if which == cm.JET else (15.0, 2)
+    tab = np.array(cm.TABLES[which], np.float32)
+    for k in range(-n, n + 1):
+        at = cm.colormap(which, np.float32(k / s))
+        assert np.allclose(at, tab[k + n], atol=1e-6)
+        below = cm.colormap(which, np.float32((k - 1e-4) / s))
+        if k > -n:
+            assert np.allclose(below, tab[k + n], atol=2e-4)
+    far = cm.colormap(which, np.array([-10.0, 10.0], np.float32))
+    assert np.array_equal(far[0], tab[0]) and np.array_equal(far[1], tab[2 * n])
+    mid = cm.colormap(which, np.float32(0.5 / s))        # halfway through block 0
+    assert np.allclose(mid, 0.5 * (tab[n] + tab[n + 1]), atol=1e-6)
+    assert not cm.colormap(which, np.array([0.01, 0.2], np.float32), np.array([1, 1])).any()
