@@ -5,9 +5,10 @@ from ._native import (BOUNDARY_REFERENCE, BOUNDARY_ZERO_GRADIENT, PRECISION_F16S
                       PRECISION_F32, LbmError, load)
 from .batch import Batch, gen_batch, gen_shapes
 from .f64 import LBM64
-from .lbm import LBM, SummaryStat, gen_circles, points_to_pairs, splitmix64
+from . import shapes
+from .lbm import LBM, ColorMap, SummaryStat, gen_circles, points_to_pairs, save_ppm, splitmix64, to_rgb8
 
 __all__ = [
-    "LBM", "LBM64", "Batch", "gen_batch", "gen_shapes", "SummaryStat", "LbmError", "gen_circles", "points_to_pairs", "splitmix64", "load",
+    "LBM", "LBM64", "shapes", "Batch", "ColorMap", "save_ppm", "to_rgb8", "gen_batch", "gen_shapes", "SummaryStat", "LbmError", "gen_circles", "points_to_pairs", "splitmix64", "load",
     "PRECISION_F32", "PRECISION_F16S", "BOUNDARY_REFERENCE", "BOUNDARY_ZERO_GRADIENT",
 ]

@@ -8,7 +8,8 @@ reference (src/lbm.rs)      here
 ==========================  ===============================================
 ``LBM::new(d, omega, x, y)``  ``LBM(omega, x, y)`` (``lbm_create``)
 ``iterate(d, n)``             ``iterate(n)`` — n steps + the summary field
-``draw_shape(d, shape)``      ``draw_shape(points)`` — (x, y, solid) triples
+``draw_shape(d, shape)``      ``draw_shape(shape)`` — a shapes.Shape or points
+``curl/chaos/welcome_barrier`` ``curl_barrier()`` etc. (shapes.py presets)
 ``update_omega_buffer``       ``update_omega_buffer(omega)``
 ``reset_barrier``             ``reset_barrier()``
 ``reset_to_equilibrium``      ``reset_to_equilibrium()``
@@ -16,6 +17,7 @@ reference (src/lbm.rs)      here
 ``single_cell(d, k)``         ``single_cell(k)``
 ``set_summary(stat)``         ``set_summary(stat)``
 ``rerender`` (summary part)   ``calculate_summary()``
+``color_map`` + summary        ``render(stat, color_map)`` — (ny, nx, 3) colours
 ``get_compute_num``           ``get_compute_num()``
 ==========================  ===============================================
 
@@ -30,6 +32,7 @@ from enum import IntEnum
 import numpy as np
 
 from . import _native as N
+from . import shapes
 
 
 class SummaryStat(IntEnum):
@@ -41,8 +44,30 @@ class SummaryStat(IntEnum):
     Speed = 4
 
 
+class ColorMap(IntEnum):
+    """Order of the reference enum (src/lbm.rs:20-24)."""
+    Inferno = 0
+    Viridis = 1
+    Jet = 2
+
+
 def _f32p(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def to_rgb8(rgb) -> np.ndarray:
+    """(..., 3) colours in [0, 1] -> uint8 (round to nearest)."""
+    return np.clip(np.rint(np.asarray(rgb, np.float32) * 255.0), 0, 255).astype(np.uint8)
+
+
+def save_ppm(path, rgb) -> None:
+    """Write an (H, W, 3) float colour plane (lbm_render's output) as a binary
+    PPM image, row 0 at the top as the reference's render pass draws it."""
+    img = to_rgb8(rgb)
+    h, w, _ = img.shape
+    with open(path, "wb") as fh:
+        fh.write(f"P6 {w} {h} 255\n".encode())
+        fh.write(np.ascontiguousarray(img).tobytes())
 
 
 def points_to_pairs(points, nx: int) -> np.ndarray:
@@ -76,6 +101,7 @@ class LBM:
         self.precision = precision
         self.boundary = boundary
         self.summary_stat = SummaryStat.Curl
+        self.color_map = ColorMap.Jet  # the reference's default (lbm.rs:1024)
 
     # -- lifecycle ---------------------------------------------------------------
     def close(self):
@@ -138,9 +164,24 @@ class LBM:
     def reset_barrier(self):
         self._chk(self._lib.lbm_reset_barrier(self._h))
 
-    def draw_shape(self, points):
-        """Paint (x, y, solid) points; see points_to_pairs."""
-        self.paint(points_to_pairs(points, self.x))
+    def draw_shape(self, shape):
+        """draw_shape (lbm.rs:1337-1339): paint a shapes.Shape, or any iterable
+        of (x, y, solid) points; see points_to_pairs."""
+        if isinstance(shape, shapes.Shape):
+            shape = shapes.ordered_points(shape)
+        self.paint(points_to_pairs(shape, self.x))
+
+    def curl_barrier(self):
+        """curl_barrier (lbm.rs:1367-1370)."""
+        self.draw_shape(shapes.curl_barrier(self.x, self.y))
+
+    def chaos_barrier(self):
+        """chaos_barrier (lbm.rs:1372-1386)."""
+        self.draw_shape(shapes.chaos_barrier(self.x, self.y))
+
+    def welcome_barrier(self):
+        """welcome_barrier (lbm.rs:1388-1480)."""
+        self.draw_shape(shapes.welcome_barrier(self.x, self.y))
 
     def paint(self, pairs):
         a = np.ascontiguousarray(np.asarray(pairs, np.uint32).ravel())
@@ -154,6 +195,18 @@ class LBM:
         out = np.empty((self.y, self.x), np.float32)
         self._chk(self._lib.lbm_summary(self._h, kind, _f32p(out)))
         return out
+
+    def render(self, stat=None, color_map=None) -> np.ndarray:
+        """calculate_summary + color_map (lbm.rs:1104-1110 without the draw):
+        (ny, nx, 3) f32 colours of the current summary field."""
+        kind = int(self.summary_stat if stat is None else stat)
+        cmap = int(self.color_map if color_map is None else color_map)
+        out = np.empty((self.y, self.x, 3), np.float32)
+        self._chk(self._lib.lbm_render(self._h, kind, cmap, _f32p(out)))
+        return out
+
+    def rerender(self) -> np.ndarray:
+        return self.render()
 
     def get_compute_num(self) -> int:
         n = ctypes.c_uint64()

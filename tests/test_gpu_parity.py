@@ -380,3 +380,60 @@ def test_autotune_leaves_the_state_untouched():
     b.step(20)
     assert same_bits(a.read_f(), b.read_f())
     assert same_bits(a.moments(True)[0], b.moments(True)[0])
+
+
+@pytest.mark.parametrize("cmap", [0, 1, 2])
+def test_render_colours_equal_colormap_restatement(cmap):
+    """lbm_render: the summary plane mapped to colours in the same pass equals
+    the CPU restatement of color_map/*.wgsl bit for bit, for every summary."""
+    print(f"case: render colormap {cmap}", flush=True)
+    from lbm_amd import LBM
+    from oracle import colormap_ref
+    mask = circle_mask(X1, Y1, [(80, 80, 16)])
+    g = LBM(OMEGA1, X1, Y1)
+    g.mask = mask
+    g.custom_speed(U1)
+    g.step(300)
+    for stat in range(5):
+        rgb = g.render(stat, cmap)
+        plane = g.calculate_summary(stat)
+        exp = colormap_ref.colormap(cmap, plane, mask)
+        assert same_bits(rgb, exp), f"stat {stat}"
+    g.close()
+
+
+def test_driver_loop_writes_frames(tmp_path):
+    """The command-line driver: frames of N steps, rendered and written as PPM."""
+    from lbm_amd import LBM, run
+    rc = run.main(["--nx", "128", "--ny", "64", "--circle", "30,32,6", "--frames", "3", "--steps-per-frame", "5",
+                   "--summary", "speed", "--colormap", "viridis", "--barrier", "curl",
+                   "--out", str(tmp_path)])
+    assert rc == 0
+    frames = sorted(p.name for p in tmp_path.iterdir())
+    assert frames == ["frame_0000.ppm", "frame_0001.ppm", "frame_0002.ppm"]
+    data = (tmp_path / "frame_0002.ppm").read_bytes()
+    assert data.startswith(b"P6 128 64 255\n") and len(data) == len(b"P6 128 64 255\n") + 128 * 64 * 3
+    g = LBM(1.0 / 0.56, 128, 64)
+    assert g.color_map == 2  # jet, the reference's default
+    g.close()
+
+
+@pytest.mark.parametrize("preset", ["curl_barrier", "chaos_barrier", "welcome_barrier"])
+def test_barrier_presets_paint_their_shapes(preset):
+    """curl/chaos/welcome_barrier (lbm.rs:1367-1480): the painted mask equals the
+    starting mask with the shape's points set, and stepping it matches refcpu."""
+    print(f"case: preset {preset}", flush=True)
+    from lbm_amd import shapes
+    g, r = make_pair(X1, Y1, OMEGA1)
+    m0 = g.mask.copy()
+    getattr(g, preset)()
+    exp = m0.copy()
+    for x, y, v in getattr(shapes, preset)(X1, Y1).points:
+        exp[y, x] = 1 if v else 0
+    got = g.mask
+    assert np.array_equal(got, exp)
+    r.set_mask(got.astype(np.uint32))
+    g.step(50)
+    r.step(50)
+    assert_state_equal(g, r, preset)
+    g.close()
