@@ -7,7 +7,13 @@
 Each frame runs `steps-per-frame` steps (lib.rs:108-204: COMPUTE_PER_RENDER
 steps per redraw, or one step in step mode), then the summary field through
 the colour map (lbm.rs:1065-1074), optionally written as a PPM image, and
-prints the step count, MLUPS of the frame and the FP64 mass.  Obstacles:
+prints the step count, MLUPS of the frame and the FP64 mass.
+
+    python -m lbm_amd.run --nx 400 --ny 160 --script controls.txt
+    python -m lbm_amd.run --nx 400 --ny 160 --script -     # commands from stdin
+
+runs the reference's interactive controls instead (lbm_amd.interaction:
+pointer strokes, undo, presets, pause and step mode), one command per line.  Obstacles:
 circles (--circle x,y,r, repeatable), a seeded set (--circles n) and/or a
 barrier preset (--barrier, lbm.rs:1367-1480); fluid
 presets as the reference's (lib.rs:155-164); viscosity as the slider's
@@ -47,6 +53,27 @@ def build_mask(a) -> np.ndarray:
     return mask
 
 
+def run_script(a) -> int:
+    from .interaction import Session
+    prec = PRECISION_F32 if a.precision == "f32" else PRECISION_F16S
+    s = Session(a.nx, a.ny, 1.0 / (3.0 * a.viscosity + 0.5), precision=prec, u_init=a.speed)
+    fh = sys.stdin if a.script == "-" else open(a.script)
+    try:
+        for n, line in enumerate(fh, 1):
+            try:
+                frame = s.execute(line)
+            except (ValueError, KeyError, IndexError, AttributeError) as e:
+                print(f"line {n}: {e}", file=sys.stderr, flush=True)
+                return 2
+            if frame is not None:
+                print(f"line {n}: step {s.lbm.get_compute_num()}  mass {s.lbm.conservation()[0]:.6f}", flush=True)
+    finally:
+        if fh is not sys.stdin:
+            fh.close()
+        s.close()
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--nx", type=int, default=400)
@@ -67,7 +94,10 @@ def main(argv=None) -> int:
     ap.add_argument("--summary", choices=[s.name.lower() for s in SummaryStat], default="curl")
     ap.add_argument("--colormap", choices=[c.name.lower() for c in ColorMap], default="jet")
     ap.add_argument("--out", default=None, help="directory for frame_NNNN.ppm images")
+    ap.add_argument("--script", default=None, help="control commands, one per line ('-' = stdin)")
     a = ap.parse_args(argv)
+    if a.script:
+        return run_script(a)
 
     prec = PRECISION_F32 if a.precision == "f32" else PRECISION_F16S
     g = LBM(1.0 / (3.0 * a.viscosity + 0.5), a.nx, a.ny, precision=prec, u_init=a.speed)

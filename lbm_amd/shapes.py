@@ -244,3 +244,125 @@ def welcome_barrier(x: int, y: int) -> Shape:
     for p1, p2 in segs:
         b.join(Line(p1, p2, x, y))
     return b
+
+
+# ---- pointer interaction with undo history (lib.rs:220-425) ------------------------
+
+class ClickType:
+    """Pointer modes (lib.rs:220-225)."""
+    Line, Erase, Draw, Inactive = range(4)
+
+
+class ClickHandler:
+    """Turns pointer events into shapes to paint, with a per-cell history that
+    undo rolls back (lib.rs:227-425).  ``current_blob`` holds what the next
+    redraw paints; ``undo`` returns the cells to repaint, or None.
+
+    One difference: an eraser drag validates its strokes against the grid's
+    real height (lib.rs:355 passes the width twice, so strokes near the bottom
+    edge of a wide grid reach past it)."""
+
+    def __init__(self, x: int, y: int):
+        self.x, self.y = int(x), int(y)
+        self.current_type = ClickType.Draw
+        self.line_points = []
+        self.current_blob = Blob()
+        self.current_curve = Curve()
+        self.contiguous_curve = CurveCollection()
+        self.undo_stack = []
+        self.history = {}
+
+    def clear_barrier(self) -> None:
+        self.current_curve = Curve()
+        self.contiguous_curve = CurveCollection()
+        self.undo_stack.clear()
+        self.history.clear()
+        self.current_blob.empty()
+        self.line_points.clear()
+
+    def _stroke(self, location) -> None:
+        if self.current_type == ClickType.Erase:
+            self.current_curve.erase_segment(location, self.x, self.y)
+        elif self.current_type == ClickType.Draw:
+            self.current_curve.add_segment(location, self.x, self.y)
+
+    def handle_movement(self, location) -> None:
+        self._stroke(location)
+
+    def handle_click(self, location) -> None:
+        if self.current_type == ClickType.Line:
+            self._line_click(location)
+        else:
+            self._stroke(location)
+
+    def handle_release(self) -> None:
+        if self.current_type in (ClickType.Erase, ClickType.Draw):
+            self.current_blob.join(self.current_curve)
+            done, self.current_curve = self.current_curve, Curve()
+            self.contiguous_curve.add_curve(done)
+            stroke, self.contiguous_curve = self.contiguous_curve, CurveCollection()
+            self._add_to_history(stroke)
+
+    def empty_all(self) -> None:
+        self.current_blob.empty()
+        self.current_curve.empty()
+
+    def switch_click_type(self, click_type: int) -> None:
+        self.current_type = click_type
+        self.line_points.clear()
+
+    def update(self, pressed: bool, location, click_type: int) -> None:
+        """After a redraw painted the current curve: start a new piece of it at
+        the pointer (lib.rs:302-310, 346-358).  ``click_type`` is the mode last
+        selected, which the next button event makes current."""
+        self.current_blob.empty()
+        if pressed and click_type in (ClickType.Draw, ClickType.Erase):
+            done, self.current_curve = self.current_curve, Curve()
+            self.contiguous_curve.add_curve(done)
+            if click_type == ClickType.Draw:
+                self.current_curve.add_segment(location, self.x, self.y)
+            else:
+                self.current_curve.erase_segment(location, self.x, self.y)
+
+    def undo(self):
+        if self.current_blob.is_empty() and self.current_curve.is_empty():
+            if self.undo_stack:
+                return self._remove_shape(self.undo_stack.pop())
+            return None
+        self.current_blob.empty()
+        self.current_curve.empty()
+        return None
+
+    def _line_click(self, location) -> None:
+        self.line_points.append(tuple(location))
+        if len(self.line_points) >= 2:
+            shape = Line(self.line_points[0], self.line_points[1], self.x, self.y)
+            self.current_blob.join(shape)
+            self._add_to_history(shape)
+            self.line_points.clear()
+
+    def validate_click(self, location):
+        """Pointer position -> cell (lib.rs:360-377): negative coordinates count
+        back from the far edge, coordinates past it clamp to the last cell."""
+        def one(v, n):
+            return n + v if v < 0 else (n - 1 if v >= n else v)
+        return (one(int(location[0]), self.x), one(int(location[1]), self.y))
+
+    def _add_to_history(self, shape: Shape) -> None:
+        for x, y, v in shape.points:
+            self.history.setdefault((x, y), []).append(v)
+        self.undo_stack.append(shape)
+
+    def _remove_shape(self, shape: Shape) -> Blob:
+        points = []
+        for x, y, _ in shape.points:
+            vals = self.history.get((x, y))
+            if vals is None:
+                points.append((x, y, False))
+                continue
+            vals.pop()
+            points.append((x, y, vals[-1] if vals else False))
+            if not vals:
+                del self.history[(x, y)]
+        points.sort()
+        return Blob().add(points, self.x, self.y)

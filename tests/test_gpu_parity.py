@@ -437,3 +437,47 @@ def test_barrier_presets_paint_their_shapes(preset):
     r.step(50)
     assert_state_equal(g, r, preset)
     g.close()
+
+
+def test_interactive_session_script(tmp_path):
+    """The reference's controls driven by a command script (lib.rs:108-204,
+    450-535): a barrier preset, a drawn line and its undo, step mode, pause."""
+    print("case: interaction script", flush=True)
+    from lbm_amd import shapes
+    from lbm_amd.interaction import Session
+    s = Session(X1, Y1, OMEGA1)
+    g = s.lbm
+    walls = g.mask.copy()
+    assert s.execute("preset curl") is None
+    assert s.execute("frame") is not None  # preset + 15 steps
+    assert g.get_compute_num() == 15
+    curl = walls.copy()
+    for x, y, _ in shapes.curl_barrier(X1, Y1).points:
+        curl[y, x] = 1
+    assert np.array_equal(g.mask, curl)
+    for c in ("draw-type line", "click 20 100", "click 60 120", "frame"):
+        s.execute(c)
+    line = curl.copy()
+    for x, y, _ in shapes.Line((20, 100), (60, 120), X1, Y1).points:
+        line[y, x] = 1
+    assert np.array_equal(g.mask, line) and g.get_compute_num() == 30
+    s.execute("undo")
+    s.execute("frame")
+    assert np.array_equal(g.mask, curl)
+    # step mode: a redraw steps only after "step"; paused with no change renders nothing
+    s.execute("step-mode on")
+    s.execute("frame")
+    assert g.get_compute_num() == 45  # the undo frame stepped 15
+    s.execute("step")
+    assert s.execute("frame") is not None and g.get_compute_num() == 46
+    s.execute("step-mode off")
+    s.execute("pause")
+    assert s.execute("frame") is None and g.get_compute_num() == 46
+    s.execute("colormap viridis")
+    rgb = s.execute("frame")
+    assert rgb is not None and g.get_compute_num() == 46 and g.color_map == 1
+    s.execute(f"save {tmp_path / 'f.ppm'}")
+    assert (tmp_path / "f.ppm").stat().st_size == len(b"P6 400 160 255\n") + X1 * Y1 * 3
+    with pytest.raises(ValueError):
+        s.execute("bogus 1")
+    s.close()

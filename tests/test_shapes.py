@@ -200,3 +200,82 @@ def test_welcome_uses_truncating_division():
     x_c = x_l + space
     col = [y for x, y in cells if x == x_c]
     assert min(col) == bottom - 20 - 1
+
+
+# ---- pointer interaction and undo (lib.rs:227-425) -----------------------------------
+
+def _cells(shape, value=None):
+    return {(x, y) for x, y, v in shape.points if value is None or v == value}
+
+
+def test_validate_click_wraps_negative_and_clamps_past_the_edge():
+    ch = S.ClickHandler(40, 30)
+    assert ch.validate_click((5, 7)) == (5, 7)
+    assert ch.validate_click((-1, -2)) == (39, 28)
+    assert ch.validate_click((40, 99)) == (39, 29)
+
+
+def test_line_mode_two_clicks_make_a_line_and_undo_clears_it():
+    ch = S.ClickHandler(64, 48)
+    ch.switch_click_type(S.ClickType.Line)
+    ch.handle_click((10, 10))
+    assert ch.current_blob.is_empty()
+    ch.handle_click((30, 20))
+    line = S.Line((10, 10), (30, 20), 64, 48)
+    assert ch.current_blob.points == line.points and len(ch.undo_stack) == 1
+    # a pending blob is dropped by the first undo, which returns nothing
+    assert ch.undo() is None and ch.current_blob.is_empty() and len(ch.undo_stack) == 1
+    u = ch.undo()
+    assert _cells(u) == _cells(line) and not _cells(u, True)
+    assert ch.history == {} and ch.undo() is None
+
+
+def test_draw_stroke_then_erase_over_it_undo_restores_the_drawing():
+    ch = S.ClickHandler(64, 48)
+    # a drawn stroke (10,24) -> (50,24)
+    ch.handle_click((10, 24))
+    ch.handle_movement((50, 24))
+    ch.handle_release()
+    drawn = S.Line((10, 24), (50, 24), 64, 48)
+    assert _cells(drawn) <= _cells(ch.current_blob, True)
+    ch.empty_all()
+    # an eraser stroke across it
+    ch.switch_click_type(S.ClickType.Erase)
+    ch.handle_click((30, 10))
+    ch.handle_movement((30, 40))
+    ch.handle_release()
+    erased = _cells(ch.current_blob, False)
+    assert (30, 24) in erased
+    ch.empty_all()
+    u = ch.undo()
+    # cells the eraser took from the drawing come back solid, the rest clear
+    for x, y, v in u.points:
+        assert v == ((x, y) in _cells(drawn) or (x, y) == (10, 24))
+    assert (30, 24, True) in u.points
+    u2 = ch.undo()
+    assert not _cells(u2, True) and ch.history == {}
+
+
+def test_update_starts_a_new_piece_of_the_stroke_at_the_pointer():
+    ch = S.ClickHandler(64, 48)
+    ch.handle_click((5, 5))
+    ch.handle_movement((15, 9))
+    ch.current_blob.join(ch.current_curve)
+    ch.update(True, (15, 9), S.ClickType.Draw)
+    assert ch.current_blob.is_empty()
+    assert ch.current_curve.points == {(15, 9, True)} and ch.current_curve.last_point == (15, 9)
+    assert (5, 5, True) in ch.contiguous_curve.points
+    ch.handle_movement((25, 9))
+    ch.handle_release()
+    # the whole drag is one undo step
+    assert len(ch.undo_stack) == 1
+    assert {(5, 5), (15, 9), (25, 9)} <= _cells(ch.undo_stack[0])
+
+
+def test_clear_barrier_forgets_history():
+    ch = S.ClickHandler(32, 32)
+    ch.switch_click_type(S.ClickType.Line)
+    ch.handle_click((1, 1))
+    ch.handle_click((9, 9))
+    ch.clear_barrier()
+    assert ch.undo_stack == [] and ch.history == {} and ch.current_blob.is_empty()
