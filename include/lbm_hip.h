@@ -139,12 +139,13 @@ int lbm_step(lbm_grid *g, uint32_t n);
 int lbm_sync(lbm_grid *g);
 
 /* Step-kernel variants (no reference counterpart): how the fused step moves
- * memory -- cells per thread and non-temporal access.  All variants compute the
- * same bits.  lbm_autotune times each for `steps` steps on the grid's own
- * state, restores that state exactly, selects the fastest, and returns it in
- * *best and the per-step time of every variant in ms[LBM_NUM_VARIANTS] (either
- * pointer may be NULL). */
-#define LBM_NUM_VARIANTS 8
+ * memory -- cells per thread, non-temporal access, block order, or (8, 9) two
+ * steps per launch through LDS, which reads and writes the state once per two
+ * steps.  All variants compute the same bits.  lbm_autotune times each for
+ * `steps` steps on the grid's own state, restores that state exactly, selects
+ * the fastest, and returns it in *best and the per-step time of every variant
+ * in ms[LBM_NUM_VARIANTS] (either pointer may be NULL). */
+#define LBM_NUM_VARIANTS 10
 int lbm_set_variant(lbm_grid *g, int variant);
 int lbm_get_variant(const lbm_grid *g, int *variant);
 int lbm_autotune(lbm_grid *g, uint32_t steps, int *best, float *ms);

@@ -18,7 +18,7 @@ LBM_ENOMEM = -3
 LBM_ENONFINITE = -4
 LBM_ENODEV = -5
 
-NUM_VARIANTS = 8
+NUM_VARIANTS = 10
 BATCH_NUM_VARIANTS = 6
 
 PRECISION_F32 = 0

@@ -71,6 +71,203 @@ __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, type
     if (OP & OP_REDUCE) block_sums(s0, s1, s2, partials, blockIdx.x);
 }
 
+// ---- two steps per launch -------------------------------------------------------
+//
+// P-form -> P-form twice in one pass over HBM.  Inputs A = buf[cur] (C_n) and
+// B = buf[1-cur]; outputs A2 := C_{n+2} and B2 := C_{n+1}, in two other buffers
+// (a block's halo reads its neighbours' inputs, so nothing is updated in place).
+// A block owns a strip of kBlock - 2 columns and `rows` rows; thread t works on
+// column x0 - 1 + t, so threads 0 and kBlock - 1 compute the strip's halo
+// columns.  Walking down the strip, phase 1 computes C_{n+1} = collide(S_{n+1})
+// of the next row, including the halo, into a ring of four rows in LDS, and
+// phase 2 pulls S_{n+2} of the current row from the ring and stores
+// collide(S_{n+2}) to A2.  C_{n+1} goes to B2 where the next step reads it
+// (cells the stream does not write) and, in the last pair of a call
+// (FINAL), everywhere.  The outlet column's wrapped pulls read column 0, which
+// belongs to another strip; its C_{n+1} (column 0 is never streamed) comes
+// from k_pair_col0, launched first.  Same per-cell operations as k_cell, so
+// the same bits.
+constexpr int kRing = 4;
+
+template <class St, int MODE>
+__global__ void k_pair_col0(const typename St::T *A, const typename St::T *B, const uint8_t *__restrict__ flags,
+                            const uint8_t *__restrict__ mask, Geom g, typename St::T *__restrict__ col0) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= g.Y) return;
+    float f[9];
+    gather_cell<St, MODE, true>(A, B, flags, mask, g, 0, r, (size_t)r * g.pitch, (size_t)r * g.ys, f);
+    St::collide(f, g.omega);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) col0[(size_t)k * g.Y + r] = St::store(f[k]);
+}
+
+// Barrier for LDS hand-off only: waits for this wave's LDS operations, not for
+// its global loads, so the next row's pulls stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <class St, int MODE, bool FINAL>
+__global__ __launch_bounds__(kBlock) void k_pair(const typename St::T *A, const typename St::T *B,
+                                                 typename St::T *__restrict__ A2, typename St::T *__restrict__ B2,
+                                                 const typename St::T *__restrict__ col0,
+                                                 const uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask,
+                                                 Geom g, float *__restrict__ mom, int nstrips, int rows, int nblocks,
+                                                 int remap) {
+    using T = typename St::T;
+    // C_{n+1} of four rows: the eight moving populations (the rest population
+    // is only read by its own cell, which keeps it in a register)
+    __shared__ T ring[kRing][8][kBlock];
+    const long long ks = g.ks, ps = g.pstride;
+    const int L = remap ? xcd_remap(blockIdx.x, nblocks) : (int)blockIdx.x;
+    const int chunk = L / nstrips;
+    const int t = (int)threadIdx.x;
+    const int x = (L - chunk * nstrips) * (kBlock - 2) - 1 + t;
+    const int r0 = chunk * rows;
+    const int r1 = min(r0 + rows, g.Y);
+    const bool in_grid = x >= 0 && x < g.X;
+    const bool owner = t >= 1 && t <= kBlock - 2 && x < g.X;
+    // cells whose pulls leave the row (column 0, the outlet column) take the
+    // general per-cell path, as in the one-step kernel
+    const bool edge = x == 0 || x + 1 >= g.X;
+    // lane columns of the pulls, clamped so every lane loads in bounds (lanes
+    // outside the grid load and discard)
+    const unsigned xc = (unsigned)min(max(x, 0), g.X - 1);
+    const unsigned xw = (unsigned)max(x - 1, 0);
+    const unsigned xe = (unsigned)min(x + 1, g.pitch - 1);
+
+    // phase 1, part 1: the nine pulls of row r, issued at clamped addresses
+    // (the rows are uniform: clamped once per block); patched in part 2
+    auto load1 = [&](int r, T v[9], uint32_t &fl) {
+        if (r < 0 || r >= g.Y) return;
+        fl = flags[(size_t)r * g.pitch + xc];
+        const T *up = A + (size_t)min(r + 1, g.Y - 1) * g.ys;  // cy = -1 pulls from r + 1
+        const T *mid = A + (size_t)r * g.ys;
+        const T *dn = A + (size_t)max(r - 1, 0) * g.ys;        // cy = +1 pulls from r - 1
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const T *row = cy(k) < 0 ? up : (cy(k) > 0 ? dn : mid);
+            const unsigned col = cx(k) < 0 ? xe : (cx(k) > 0 ? xw : xc);
+            v[k] = row[(size_t)k * ks + col];
+        }
+    };
+    // phase 1, part 2: S_{n+1} of row r (bounce-back, unstreamed cells, edges),
+    // C_{n+1} = collide into the ring; returns the rest population
+    auto finish1 = [&](int r, const T v[9], uint32_t fl) -> T {
+        if (!in_grid || r < 0 || r >= g.Y) return T{};
+        const size_t own = (size_t)r * g.pitch + x;
+        const size_t sown = (size_t)r * g.ys + x;
+        float f[9];
+        bool streamed = true;
+        if (edge) {
+            streamed = gather_cell<St, MODE, true>(A, B, flags, mask, g, x, r, own, sown, f);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) f[k] = St::load(v[k]);
+            if (fl == kSlowCell) {
+                streamed = mask[own] != 1 && r < g.Y - 1;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    if (k == C) continue;
+                    f[k] = St::load(streamed ? A[(size_t)opp(k) * ks + sown] : B[(size_t)k * ks + sown]);
+                }
+            } else if (fl != 0) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    if (k == C) continue;
+                    if ((fl >> bit(k)) & 1u) f[k] = St::load(A[(size_t)opp(k) * ks + sown]);
+                }
+            }
+        }
+        St::collide(f, g.omega);
+        const bool store = owner && r >= r0 && r < r1 && (FINAL || !streamed);
+        T o[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            o[k] = St::store(f[k]);
+            if (k != C) ring[r & (kRing - 1)][bit(k)][t] = o[k];
+        }
+        if (store) {
+            T *dst = B2 + (size_t)r * g.ys;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) dst[(size_t)k * ks + x] = o[k];
+        }
+        return o[C];
+    };
+
+    // phase 2: S_{n+2} of row y from the ring, collide, store to A2; fl is
+    // the row's flag byte, read by phase 1
+    auto phase2 = [&](int y, T rest, uint32_t fl) {
+        if (!owner) return;
+        const int me = y & (kRing - 1);
+        float f[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            if (k == C) continue;
+            f[k] = St::load(ring[(y - cy(k)) & (kRing - 1)][bit(k)][t - cx(k)]);
+        }
+        f[C] = St::load(rest);
+        if (fl == kSlowCell) {
+            const size_t sown = (size_t)y * g.ys + x;
+            const bool streamed = mask[(size_t)y * g.pitch + x] != 1 && x != 0 && y < g.Y - 1;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                if (k == C) continue;
+                f[k] = streamed ? St::load(ring[me][bit(opp(k))][t]) : St::load(A[(size_t)k * ks + sown]);
+            }
+        } else if (fl != 0 || x == g.X - 1) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                if (k == C) continue;
+                if ((fl >> bit(k)) & 1u) {
+                    f[k] = St::load(ring[me][bit(opp(k))][t]);
+                } else if (x - cx(k) >= g.X) {
+                    // the outlet's pull past the last column: the flattened
+                    // index wraps to column 0 of the next row, or clamps
+                    if constexpr (MODE == LBM_BOUNDARY_REFERENCE)
+                        f[k] = St::load(col0[(size_t)k * g.Y + (y - cy(k)) + 1]);
+                    else
+                        f[k] = St::load(ring[(y - cy(k)) & (kRing - 1)][bit(k)][t]);
+                }
+            }
+        }
+        const Moments m = St::collide(f, g.omega);
+        T *dst = A2 + (size_t)y * g.ys;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(St::store(f[k]), dst + (size_t)k * ks + x);
+        if (FINAL) {
+            const size_t own = (size_t)y * g.pitch + x;
+            mom[own] = m.jx;
+            mom[ps + own] = m.jy;
+            mom[2 * ps + own] = m.rho;
+        }
+    };
+
+    // rows r0 - 1 and r0, then per row y: row y + 2's pulls issued, row y + 1
+    // finished into the ring, barrier, row y's second step; unrolled by two so
+    // the two register sets alternate instead of being copied
+    T v[9];
+    uint32_t fl = 0;
+    load1(r0 - 1, v, fl);
+    finish1(r0 - 1, v, fl);
+    load1(r0, v, fl);
+    T rest = finish1(r0, v, fl);
+    uint32_t fl_y = fl;
+    load1(r0 + 1, v, fl);
+    for (int y = r0; y < r1; ++y) {
+        T cv[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) cv[k] = v[k];
+        const uint32_t cfl = fl;
+        if (y + 2 <= r1) load1(y + 2, v, fl);
+        const T next = finish1(y + 1, cv, cfl);
+        lds_barrier();
+        phase2(y, rest, fl_y);
+        rest = next;
+        fl_y = cfl;
+    }
+}
+
 // ---- surrogate support ---------------------------------------------------------
 
 // Network input (NHWC bf16, C channels, nx / F x ny / F pixels) from the
@@ -206,6 +403,8 @@ struct lbm_grid {
     float omega = 1.25f;
     float u_in = 0.1f;  // x velocity of the last uniform reset: the inlet column's speed
     void *f[2] = {nullptr, nullptr};
+    void *spare[2] = {nullptr, nullptr};  // output sets of the two-step kernel, allocated on first use
+    void *col0 = nullptr;                 // its outlet-wrap column (9 x ny elements)
     int cur = 0;
     bool pform = false;
     uint8_t *mask = nullptr, *flags = nullptr;
@@ -243,7 +442,7 @@ struct lbm_grid {
 
     void release() {
         (void)hipSetDevice(dev);
-        for (void *p : {f[0], f[1], (void *)mask, (void *)flags,
+        for (void *p : {f[0], f[1], spare[0], spare[1], col0, (void *)mask, (void *)flags,
                         (void *)lag, (void *)now, (void *)out, (void *)dense, (void *)partials, (void *)sums,
                         (void *)snap, (void *)umark, (void *)vpart})
             if (p) (void)hipFree(p);
@@ -251,6 +450,7 @@ struct lbm_grid {
         umark = nullptr;
         vpart = nullptr;
         f[0] = f[1] = nullptr;
+        spare[0] = spare[1] = col0 = nullptr;
         mask = flags = nullptr;
         lag = now = out = dense = nullptr;
         partials = sums = nullptr;
@@ -284,6 +484,7 @@ struct lbm_grid {
     //   cells    1 1 1 2 2 4 4 4      cells    1 1 1 2 2 4 4 8
     //   nt       1 1 0 1 1 1 1 0      nt       1 1 0 1 1 1 1 1
     //   remap    1 0 0 1 0 1 0 0      remap    1 0 0 1 0 1 0 0
+    // and 8, 9: two steps per launch (k_pair, see pair()).
     template <class St, int MODE, bool PFORM, int OP>
     void launch_cell(const void *src, void *dst, float *mom, double *part) {
         constexpr bool f32 = sizeof(typename St::T) == 4;
@@ -297,12 +498,68 @@ struct lbm_grid {
             case 4: return go<St, MODE, PFORM, OP, 2, true>(src, dst, mom, part, 0);
             case 5: return go<St, MODE, PFORM, OP, 4, true>(src, dst, mom, part, 1);
             case 6: return go<St, MODE, PFORM, OP, 4, true>(src, dst, mom, part, 0);
-            default:
+            case 7:
                 if constexpr (f32) return go<St, MODE, PFORM, OP, 4, false>(src, dst, mom, part, 0);
                 else return go<St, MODE, PFORM, OP, 8, true>(src, dst, mom, part, 0);
+            default:  // single steps of the two-step variants: variant 0 (f32) / 4 (f16s)
+                if constexpr (f32) return go<St, MODE, PFORM, OP, 1, true>(src, dst, mom, part, 1);
+                else return go<St, MODE, PFORM, OP, 2, true>(src, dst, mom, part, 0);
             }
         }
         go<St, MODE, PFORM, OP, (PFORM && !f32 ? 2 : 1), true>(src, dst, mom, part, 0);
+    }
+
+    // Variants 8, 9: two P-form steps per launch (k_pair), blocks of 8 or 16 rows
+    // (measured at 8192 x 4096: 2 / 4 / 8 / 16 / 32 rows 362 / 312 / 303 / 314 /
+    // 333 us per step).
+    static constexpr int kPairFirst = 8;
+    static bool pair_variant(int v) { return v >= kPairFirst; }
+    int pair_rows() const { return 8 << (variant - kPairFirst); }
+
+    template <class St, int MODE>
+    void launch_pair(bool fin) {
+        using T = typename St::T;
+        if (!spare[0]) {
+            spare[0] = dalloc<char>(set_bytes());
+            spare[1] = dalloc<char>(set_bytes());
+            col0 = dalloc<char>(9 * (size_t)Y * esize);
+            // the rows' padding is never written by this kernel: keep it defined
+            HIPCHK(hipMemsetAsync(spare[0], 0, set_bytes(), stream));
+            HIPCHK(hipMemsetAsync(spare[1], 0, set_bytes(), stream));
+        }
+        const T *A = (const T *)f[cur], *B = (const T *)f[1 - cur];
+        T *A2 = (T *)spare[0], *B2 = (T *)spare[1];
+        if (MODE == LBM_BOUNDARY_REFERENCE) {
+            hipLaunchKernelGGL((k_pair_col0<St, MODE>), dim3((Y + 255) / 256), dim3(256), 0, stream, A, B, flags,
+                               mask, geom, (T *)col0);
+            HIPCHK(hipGetLastError());
+        }
+        const int rows = pair_rows();
+        const int nstrips = ((int)X + kBlock - 3) / (kBlock - 2);
+        const int nblocks = nstrips * (((int)Y + rows - 1) / rows);
+        if (fin)
+            hipLaunchKernelGGL((k_pair<St, MODE, true>), dim3(nblocks), dim3(kBlock), 0, stream, A, B, A2, B2,
+                               (const T *)col0, flags, mask, geom, lag, nstrips, rows, nblocks, 1);
+        else
+            hipLaunchKernelGGL((k_pair<St, MODE, false>), dim3(nblocks), dim3(kBlock), 0, stream, A, B, A2, B2,
+                               (const T *)col0, flags, mask, geom, lag, nstrips, rows, nblocks, 1);
+        HIPCHK(hipGetLastError());
+        spare[0] = f[cur];
+        spare[1] = f[1 - cur];
+        f[cur] = A2;
+        f[1 - cur] = B2;
+    }
+
+    // two P-form steps; `fin`: the last launch of a step call (writes every
+    // cell of the other set and the renderer's moments)
+    void pair(bool fin) {
+        if (precision == LBM_PRECISION_F32) {
+            if (boundary == LBM_BOUNDARY_REFERENCE) launch_pair<StoreF32, LBM_BOUNDARY_REFERENCE>(fin);
+            else launch_pair<StoreF32, LBM_BOUNDARY_ZERO_GRADIENT>(fin);
+        } else {
+            if (boundary == LBM_BOUNDARY_REFERENCE) launch_pair<StoreF16S, LBM_BOUNDARY_REFERENCE>(fin);
+            else launch_pair<StoreF16S, LBM_BOUNDARY_ZERO_GRADIENT>(fin);
+        }
     }
 
     template <bool PFORM, int OP>
@@ -369,6 +626,13 @@ struct lbm_grid {
     void step(uint32_t n) {
         for (uint32_t i = 0; i < n; ++i) {
             const bool last = i + 1 == n;
+            if (pform && pair_variant(variant) && n - i >= 2) {
+                // between launches of one call the other set is current only on
+                // cells the stream does not write, the only ones read from it
+                pair(i + 2 == n);
+                ++i;
+                continue;
+            }
             if (!pform) {
                 if (last)
                     cell<false, OP_COLLIDE | OP_MOMENTS>(f[cur], f[cur], lag, nullptr);

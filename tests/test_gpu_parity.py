@@ -338,7 +338,7 @@ def test_full_size_properties_8192x4096():
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("variant", list(range(8)))
+@pytest.mark.parametrize("variant", list(range(10)))
 def test_every_kernel_variant_gives_the_same_bits(precision, variant):
     """The step-kernel variants move memory differently and must not change a bit.
     Odd width: exercises partial vector groups, the outlet column and the slow path."""
@@ -365,6 +365,66 @@ def test_every_kernel_variant_gives_the_same_bits(precision, variant):
         assert_state_equal(g, r, f"variant {variant}")
 
 
+def both_sets(g):
+    """Both parity sets of the saved state as (2, ny, 9, nx) raw elements (the
+    rows' padding, which holds no state, cut off)."""
+    blob = np.frombuffer(g.save_state(), np.uint8)
+    es = 4 if g.precision == 0 else 2
+    pitch = (g.x + 63) // 64 * 64
+    ps = (pitch * g.y + 255) // 256 * 256
+    set_bytes = 9 * pitch * g.y * es
+    hdr = blob.size - (2 * set_bytes + 17 * ps)
+    sets = blob[hdr:hdr + 2 * set_bytes].view(np.uint32 if es == 4 else np.uint16)
+    return sets.reshape(2, g.y, 9, pitch)[..., :g.x]
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("variant", [8, 9])
+def test_two_step_variants_whole_state_and_paint_between_calls(variant, precision, boundary):
+    """Two steps per launch: after every call of odd and even length, both sets
+    (save_state), the current state and the renderer's moments equal the
+    one-step kernel's bit for bit; a paint between calls then reads the other
+    set on newly solid cells, which the last launch of a call must have written.
+    2309 x 37: ten strips of 254 columns, row blocks cut by the grid's end."""
+    print(f"case: two-step variant {variant} precision {precision} boundary {boundary}", flush=True)
+    from lbm_amd import LBM
+    X, Y = 2309, 37
+    mask = circle_mask(X, Y, [(300, 18, 6), (1270, 10, 5), (2300, 20, 4), (254, 30, 3)])
+    grids = []
+    for v in (0, variant):
+        g = LBM(1.75, X, Y, precision=precision, boundary=boundary)
+        g.mask = mask
+        g.custom_speed(0.09)
+        g.variant = v
+        grids.append(g)
+    a, b = grids
+    calls = [1, 7, 10, 2, 1, 5, 4]
+    for i, n in enumerate(calls):
+        a.step(n)
+        b.step(n)
+        sa, sb = both_sets(a), both_sets(b)
+        if not np.array_equal(sa, sb):
+            d = np.argwhere(sa != sb)
+            pytest.fail(f"call {i} ({n} steps): {len(d)} differing values, first at (set,y,k,x)={tuple(d[0])}")
+        assert same_bits(a.moments(True)[2], b.moments(True)[2])
+        if i == 3:
+            pairs = np.array([1000 + 15 * X, 1, 1001 + 15 * X, 1, 1800 + 20 * X, 1], np.uint32)
+            a.paint(pairs)
+            b.paint(pairs)
+    assert same_bits(a.read_f(), b.read_f())
+    if precision == 0 and boundary == 0:
+        r = oracle.RefCPU(X, Y, 1.75)
+        r.set_mask(mask.astype(np.uint32))
+        r.reset_equilibrium(1.0, 0.09, 0.0)
+        r.step(sum(calls[:4]))
+        assert r.paint(pairs) == 0
+        r.step(sum(calls[4:]))
+        assert_state_equal(b, r, f"two-step variant {variant}")
+    a.close()
+    b.close()
+
+
 def test_autotune_leaves_the_state_untouched():
     from lbm_amd import LBM
     X, Y = 640, 200
@@ -374,7 +434,8 @@ def test_autotune_leaves_the_state_untouched():
     a.step(33)
     b.step(33)
     best, ms = a.autotune(5)
-    assert 0 <= best < 8 and len(ms) == 8 and all(m > 0 for m in ms)
+    from lbm_amd._native import NUM_VARIANTS
+    assert 0 <= best < NUM_VARIANTS and len(ms) == NUM_VARIANTS and all(m > 0 for m in ms)
     assert a.get_compute_num() == 33
     a.step(20)
     b.step(20)
