@@ -130,11 +130,11 @@ __global__ __launch_bounds__(kBlock) void k_pair(const typename St::T *A, const 
     // cells whose pulls leave the row (column 0, the outlet column) take the
     // general per-cell path, as in the one-step kernel
     const bool edge = x == 0 || x + 1 >= g.X;
-    // lane columns of the pulls, clamped so every lane loads in bounds (lanes
-    // outside the grid load and discard)
+    // lane columns of the pulls, clamped into the row so every lane loads in
+    // bounds (lanes outside the grid, x = -1 or x >= nx, load and discard)
     const unsigned xc = (unsigned)min(max(x, 0), g.X - 1);
-    const unsigned xw = (unsigned)max(x - 1, 0);
-    const unsigned xe = (unsigned)min(x + 1, g.pitch - 1);
+    const unsigned xw = (unsigned)min(max(x - 1, 0), g.X - 1);
+    const unsigned xe = (unsigned)min(max(x + 1, 0), g.pitch - 1);
 
     // phase 1, part 1: the nine pulls of row r, issued at clamped addresses
     // (the rows are uniform: clamped once per block); patched in part 2

@@ -425,6 +425,36 @@ def test_two_step_variants_whole_state_and_paint_between_calls(variant, precisio
     b.close()
 
 
+@pytest.mark.parametrize("X,Y", [(3, 3), (5, 4), (64, 9), (254, 17), (255, 8), (256, 3), (509, 12), (512, 25)])
+def test_two_step_variants_small_and_boundary_sizes(X, Y):
+    """Grids at and around the strip width (254 columns), the row block (8 or 16
+    rows) and the row pitch (a multiple of 64, where the east pull is clamped at
+    the last element of the row), down to the 3 x 3 minimum, both precisions and
+    outlet modes: both sets equal the one-step kernel's bit for bit."""
+    print(f"case: two-step small {X}x{Y}", flush=True)
+    from lbm_amd import LBM
+    circles = [(X // 2, Y // 2, max(1, min(X, Y) // 4))] if min(X, Y) >= 8 else []
+    mask = circle_mask(X, Y, circles)
+    for precision in (0, 1):
+        for boundary in (0, 1):
+            for variant in (8, 9):
+                grids = []
+                for v in (0, variant):
+                    g = LBM(1.6, X, Y, precision=precision, boundary=boundary)
+                    g.mask = mask
+                    g.custom_speed(0.07)
+                    g.variant = v
+                    g.step(9)
+                    g.step(6)
+                    grids.append(g)
+                a, b = grids
+                sa, sb = both_sets(a), both_sets(b)
+                assert np.array_equal(sa, sb), (precision, boundary, variant)
+                assert same_bits(a.moments(True)[0], b.moments(True)[0])
+                a.close()
+                b.close()
+
+
 def test_autotune_leaves_the_state_untouched():
     from lbm_amd import LBM
     X, Y = 640, 200
