@@ -93,7 +93,8 @@ def surrogate_leg(steps=10000):
     with a stream sync), the velocity rel-L2 of the rollout against the plain
     LBM at the end and after 1 and 10 cycles, and the same short-horizon
     errors for the network-free ablation (the change predicted as zero:
-    re-equilibration + correctors only)."""
+    re-equilibration + correctors only).  Both sides step with the autotuned
+    step kernel."""
     import numpy as np
 
     from lbm_amd import LBM, gen_circles
@@ -109,6 +110,7 @@ def surrogate_leg(steps=10000):
         g.mask = mask
         g.custom_speed(u)
         g.step(2000)
+        g.autotune(20)  # the fastest step kernel for both sides (state restored)
         return g
     net = UNet(1, Y // 2, X // 2, seed=1)
     params = load_params(SURROGATE_WEIGHTS)
