@@ -12,13 +12,12 @@ Printed: ONE JSON line with the FP32 MLUPS as `value`, the roofline of the
 fused step kernel (72 algorithmic bytes per cell update times the updates one
 launch makes -- two for the two-step variants, which read and write the state
 once per pair of steps -- over the average launch duration from HIP events
-recorded on the grid's own stream), the same for
-FP16-shifted storage (36 B per cell update), both again at 16384 x 8192
-(large_grid_leg), the batched configuration's
-MLUPS (batched_leg), the surrogate rollout's speed-up
-and velocity error against plain FP32 LBM (surrogate_leg), and the CPU
-baseline: the oracle's faithful f32 restatement of the reference's eight
-passes, timed on a bounded sample of the same workload on the host cores.
+recorded on the grid's own stream), the same for FP16-shifted storage (36 B
+per cell update), both again at 4096 x 2048 and 16384 x 8192 (grid_leg), the
+batched configuration's MLUPS (batched_leg, 5000 steps), the surrogate
+rollout's speed-up and velocity error against plain FP32 LBM (surrogate_leg),
+and the CPU baseline: the oracle's faithful f32 restatement of the reference's
+eight passes, timed on a bounded sample of the same workload on the host cores.
 """
 from __future__ import annotations
 
@@ -58,7 +57,7 @@ def updates_per_launch(variant):
 
 
 def run_gpu(precision, X, Y, mask, steps, warmup, omega, u_in, variant=None):
-    from lbm_amd import LBM
+    from lbm_amd import LBM, LbmError
     g = LBM(omega, X, Y, precision=precision)
     g.mask = mask
     g.custom_speed(u_in)
@@ -75,7 +74,11 @@ def run_gpu(precision, X, Y, mask, steps, warmup, omega, u_in, variant=None):
     t0 = time.perf_counter()
     dev_ms = g.step_timed(steps)  # waits for the last launch
     wall = time.perf_counter() - t0
-    cons = g.conservation()
+    try:
+        cons = g.conservation()
+    except LbmError as e:  # the flow left BGK's stable range: timing stands, sums do not
+        log(f"conservation: {e}")
+        cons = None
     g.close()
     return wall, dev_ms, cons, tuned
 
@@ -141,25 +144,26 @@ def surrogate_leg(steps=10000):
     return out
 
 
-def large_grid_leg(steps=100):
-    """FP32 and FP16-shifted MLUPS at 16384 x 8192 (the large grid of the FP16
-    configuration), same obstacle generator and parameters as the main line,
-    variants autotuned."""
+def grid_leg(X, Y, steps):
+    """FP32 and FP16-shifted MLUPS at another grid size (4096 x 2048, the small
+    grid of the plain FP32 configuration; 16384 x 8192, the large grid of the
+    FP16 configuration), same obstacle generator and parameters as the main
+    line, variants autotuned."""
     from lbm_amd import PRECISION_F16S, PRECISION_F32, gen_circles
-    X, Y = 16384, 8192
     mask = gen_circles(X, Y, SEED, 200, 8, 64)
     out = {"workload": f"{X}x{Y}, 200 circles r 8-64 (seed {SEED}), tau 0.53, u_in 0.08"}
     for name, prec, bpc in (("f32", PRECISION_F32, 72), ("f16s", PRECISION_F16S, 36)):
-        wall, dev_ms, _, tuned = run_gpu(prec, X, Y, mask, steps, 10, 1.0 / 0.53, 0.08)
+        wall, dev_ms, cons, tuned = run_gpu(prec, X, Y, mask, steps, 10, 1.0 / 0.53, 0.08)
         k = dev_ms / 1e3 / steps
         out[name] = {"mlups": round(X * Y / k / 1e6, 1), "us_per_step": round(k * 1e6, 2),
                      "achieved_gbs": round(bpc * X * Y / k / 1e9, 1), "variant": tuned["variant"],
-                     "steps_per_launch": updates_per_launch(tuned["variant"])}
+                     "steps_per_launch": updates_per_launch(tuned["variant"]), "steps": steps,
+                     "state_finite": cons is not None}
     out["f16s_speedup_vs_f32"] = round(out["f16s"]["mlups"] / out["f32"]["mlups"], 3)
     return out
 
 
-def batched_leg(n_domains=512, steps=200):
+def batched_leg(n_domains=512, steps=5000):
     """The batched configuration: 512 independent domains (sizes 128x64 ..
     1024x512 in multiples of 32, 1-12 circles/rectangles, tau in [0.52, 0.8],
     u_in in [0.02, 0.1], all from seed 2024) stepped together; tile variant
@@ -218,7 +222,7 @@ def cpu_baseline(X, Y, mask, omega, u_in, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--nx", type=int, default=8192)
     ap.add_argument("--ny", type=int, default=4096)
@@ -289,7 +293,7 @@ def main():
             "cell_updates_per_launch": cells * upl,
             "kernel_avg_us": round(kern_s * upl * 1e6, 2),
         },
-        "fp64_mass_momentum": [float(c) for c in cons],
+        "fp64_mass_momentum": [float(c) for c in cons] if cons is not None else None,
         "autotune": tuned,
     }
 
@@ -311,13 +315,15 @@ def main():
                          "cell_updates_per_launch": cells * u16, "kernel_avg_us": round(k16 * u16 * 1e6, 2),
                          "traffic": read_traffic("traffic_f16s_8192x4096.txt")
                          if (X, Y) == (8192, 4096) and u16 == 1 else None},
-            "fp64_mass_momentum": [float(c) for c in c16],
+            "fp64_mass_momentum": [float(c) for c in c16] if c16 is not None else None,
             "autotune": tuned16,
         }
 
     if not args.no_large and not args.no_f16:
-        log("16384x8192: f32 and f16s")
-        out["large_grid"] = large_grid_leg()
+        log("4096x2048 and 16384x8192: f32 and f16s")
+        out["small_grid"] = grid_leg(4096, 2048, 2000)
+        log(f"small grid: {out['small_grid']}")
+        out["large_grid"] = grid_leg(16384, 8192, 2000)
         log(f"large grid: {out['large_grid']}")
 
     if not args.no_batched:
