@@ -21,8 +21,8 @@ import time
 import numpy as np
 
 from .batch import Batch, gen_batch, gen_shapes
-from .surrogate import (D_RHO, D_U, S_RHO, S_U, UNet, input_from_pooled, loss_weights, pool_fields, save_params,
-                        target_from_pooled)
+from .surrogate import (D_RHO, D_U, S_RHO, S_U, UNet, input_from_pooled, load_params, loss_weights, pool_fields,
+                        save_params, target_from_pooled)
 
 
 def velocity(jx, jy, rho):
@@ -148,10 +148,15 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--time-budget", type=float, default=None, help="stop after this many seconds")
     ap.add_argument("--out", default="surrogate.npy", help="parameters, bfloat16 bit patterns (.npy)")
+    ap.add_argument("--init", default=None, help="start from these parameters (.npy, as --out writes)")
     a = ap.parse_args()
     size = a.size or 256 * a.factor
+    net = None
+    if a.init:
+        net = UNet(a.minibatch, size // a.factor, size // a.factor, seed=a.seed)
+        net.set_params(load_params(a.init))
     net, losses = train(a.iters, a.minibatch, a.lr, a.domains, size, a.factor, seed=a.seed,
-                        time_budget=a.time_budget)
+                        time_budget=a.time_budget, net=net)
     save_params(a.out, net.get_params())
     print(f"saved {a.out}; loss {losses[:10].mean():.4e} -> {losses[-10:].mean():.4e}; "
           f"S_RHO={S_RHO} S_U={S_U} D_RHO={D_RHO} D_U={D_U} factor={a.factor}")
