@@ -232,6 +232,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=5000, help="timed steps of the batched leg")
     ap.add_argument("--variant", type=int, default=None, help="FP32 kernel variant (default: autotune)")
     ap.add_argument("--f16-variant", type=int, default=None, help="FP16-shifted kernel variant (default: autotune)")
     args = ap.parse_args()
@@ -328,7 +329,7 @@ def main():
 
     if not args.no_batched:
         log("batched: 512 domains")
-        out["batched"] = batched_leg()
+        out["batched"] = batched_leg(steps=args.batched_steps)
         log(f"batched: {out['batched']}")
 
     if not args.no_surrogate:

@@ -98,3 +98,43 @@ def compare(make_grid, net, steps: int, **kw):
     err = velocity_error(g.fields(), ref, mask)
     g.close()
     return t_plain / t_ro, err, st, t_plain, t_ro
+
+
+def main(argv=None) -> int:
+    """Run the rollout on a seeded grid (the workload `bench.py` measures):
+
+        python -m lbm_amd.rollout --cycles 20 [--spinup 200] [--weights PATH]
+    """
+    import argparse
+    import os
+
+    from . import LBM, gen_circles
+    from .surrogate import UNet, load_params
+    ap = argparse.ArgumentParser(description="surrogate + LBM-corrector rollout on a seeded grid")
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--circles", type=int, default=12)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--tau", type=float, default=0.6)
+    ap.add_argument("--u-in", type=float, default=0.05)
+    ap.add_argument("--spinup", type=int, default=200)
+    ap.add_argument("--cycles", type=int, default=20)
+    ap.add_argument("--threshold", type=float, default=0.05)
+    ap.add_argument("--weights", default=os.path.join(os.path.dirname(__file__), "data", "surrogate_f2.npy"))
+    a = ap.parse_args(argv)
+    g = LBM(1.0 / a.tau, a.nx, a.ny)
+    g.mask = gen_circles(a.nx, a.ny, a.seed, a.circles, 8, 32)
+    g.custom_speed(a.u_in)
+    g.step(a.spinup)
+    net = UNet(1, a.ny // 2, a.nx // 2)
+    net.set_params(load_params(a.weights))
+    st = Rollout(g, net, threshold=a.threshold).run((64 + 8) * a.cycles)
+    g.sync()
+    print(f"cycles {st.cycles} jumps {st.jumps} fallbacks {st.fallbacks} steps {g.get_compute_num()}", flush=True)
+    net.close()
+    g.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
