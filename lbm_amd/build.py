@@ -1,4 +1,5 @@
-"""Build the gfx950 shared library in-tree: lbm_amd/liblbm_hip.so.
+"""Build the gfx950 shared library in-tree, lbm_amd/liblbm_hip.so, and the C++
+command-line driver linked against it, lbm_amd/lbm_cli.
 
     python -m lbm_amd.build
 
@@ -42,6 +43,26 @@ def sources():
     return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
+CLI = os.path.join(HERE, "lbm_cli")
+CLI_SOURCE = os.path.join(CSRC, "lbm_cli.cpp")
+
+
+def build_cli(verbose: bool = False) -> str:
+    """The C++ command-line driver (host code only, linked against the library)."""
+    cxx = os.environ.get("CXX") or shutil.which("g++") or "g++"
+    tmp = CLI + ".tmp"
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), CLI_SOURCE,
+           "-L", HERE, "-l:liblbm_hip.so", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("building lbm_cli failed")
+    os.replace(tmp, CLI)
+    return CLI
+
+
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + sources()
@@ -52,6 +73,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed")
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
 
 

@@ -148,3 +148,18 @@ def test_save_ppm_writes_a_binary_ppm(tmp_path):
     assert data.startswith(b"P6 3 2 255\n")
     body = data[len(b"P6 3 2 255\n"):]
     assert len(body) == 18 and body[:3] == bytes([255, 128, 0])
+
+
+CLI = os.path.join(os.path.dirname(__file__), "..", "lbm_amd", "lbm_cli")
+
+
+def test_cpp_cli_builds_and_parses_arguments_without_a_device():
+    """The C++ driver over include/lbm_hip.hpp: usage, argument errors reported
+    as errors (exit 1 / 2), no crash; nothing here touches a GPU."""
+    assert os.path.exists(CLI), "lbm_amd/lbm_cli is built with the library"
+    r = subprocess.run([CLI, "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "usage: lbm_cli" in r.stdout
+    r = subprocess.run([CLI, "--precision", "f8"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "--precision is f32 or f16s" in r.stderr
+    r = subprocess.run([CLI, "--bogus"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "unknown option" in r.stderr
