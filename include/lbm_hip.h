@@ -141,8 +141,8 @@ int lbm_sync(lbm_grid *g);
 /* Step-kernel variants (no reference counterpart): how the fused step moves
  * memory -- cells per thread, non-temporal access, block order, or (8, 9) two
  * steps per launch through LDS, which reads and writes the state once per two
- * steps (and allocates two more state sets on its first launch).  All variants
- * compute the same bits.  lbm_autotune times each for
+ * steps (and allocates two more state sets on its first launch, freed again
+ * when another variant is set or chosen).  All variants compute the same bits.  lbm_autotune times each for
  * `steps` steps on the grid's own state, restores that state exactly, selects
  * the fastest, and returns it in *best and the per-step time of every variant
  * in ms[LBM_NUM_VARIANTS] (either pointer may be NULL). */

@@ -550,6 +550,17 @@ struct lbm_grid {
         f[1 - cur] = B2;
     }
 
+    // the two-step kernel's output sets and column buffer (the state itself
+    // is always in f[0], f[1] between calls)
+    void release_pair_buffers() {
+        for (void *&p : spare) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+        }
+        if (col0) (void)hipFree(col0);
+        col0 = nullptr;
+    }
+
     // two P-form steps; `fin`: the last launch of a step call (writes every
     // cell of the other set and the renderer's moments)
     void pair(bool fin) {
@@ -1015,7 +1026,9 @@ int lbm_step_timed(lbm_grid *g, uint32_t n, float *ms) {
 int lbm_set_variant(lbm_grid *g, int variant) {
     return guarded(g, [&] {
         check(variant >= 0 && variant < LBM_NUM_VARIANTS, "unknown kernel variant");
+        HIPCHK(hipStreamSynchronize(g->stream));
         g->variant = variant;
+        if (!g->pair_variant(variant)) g->release_pair_buffers();
     });
 }
 
@@ -1069,6 +1082,7 @@ int lbm_autotune(lbm_grid *g, uint32_t steps, int *best, float *ms) {
         g->pform = pform;
         g->steps = nsteps;
         g->variant = v_best;
+        if (!g->pair_variant(v_best)) g->release_pair_buffers();  // tried, not kept
         if (best) *best = v_best;
         if (ms)
             for (int v = 0; v < LBM_NUM_VARIANTS; ++v) ms[v] = t[v] / steps;

@@ -455,6 +455,24 @@ def test_two_step_variants_small_and_boundary_sizes(X, Y):
                 b.close()
 
 
+def test_switching_between_one_and_two_step_variants():
+    """Leaving a two-step variant frees its extra sets; switching back and forth
+    between calls keeps the state equal to the one-step kernel's."""
+    print("case: variant switching", flush=True)
+    from lbm_amd import LBM
+    X, Y = 300, 70
+    mask = circle_mask(X, Y, [(60, 35, 9), (200, 20, 6)])
+    a, _ = make_pair(X, Y, 1.7, mask)
+    b, _ = make_pair(X, Y, 1.7, mask)
+    for v, n in ((8, 7), (0, 5), (9, 6), (4, 3), (8, 4)):
+        a.variant = v
+        a.step(n)
+        b.step(n)
+        assert np.array_equal(both_sets(a), both_sets(b)), (v, n)
+    a.close()
+    b.close()
+
+
 def test_autotune_leaves_the_state_untouched():
     from lbm_amd import LBM
     X, Y = 640, 200
