@@ -7,8 +7,9 @@
 //   lbm_cli --nx 400 --ny 160 --viscosity 0.02 --speed 0.1 --circle 80,80,16 \
 //           --frames 4 --steps-per-frame 500
 //
-// Obstacles: --circle x,y,r (repeatable) or --circles N (seeded, lbm_gen_circles);
-// walls on rows 0 and ny-1 always.  omega = 1 / (3 nu + 1/2) (lib.rs:181-186).
+// Obstacles: --circle x,y,r (repeatable) or --circles N (seeded, lbm_gen_circles),
+// and/or a barrier preset (--barrier, include/lbm_shapes.hpp); walls on rows 0
+// and ny-1 always.  omega = 1 / (3 nu + 1/2) (lib.rs:181-186).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "lbm_hip.hpp"
+#include "lbm_shapes.hpp"
 
 namespace {
 
@@ -29,6 +31,7 @@ struct Args {
     int precision = LBM_PRECISION_F32, boundary = LBM_BOUNDARY_REFERENCE;
     uint32_t frames = 1, steps_per_frame = 15;  // COMPUTE_PER_RENDER (lib.rs:17)
     int variant = -1;                            // -1: autotune
+    std::string barrier;                         // curl, chaos or welcome (lbm.rs:1367-1480)
     std::string dump;
 };
 
@@ -37,7 +40,7 @@ void usage(FILE *to) {
                  "usage: lbm_cli [--nx N] [--ny N] [--viscosity NU] [--speed U] [--circle X,Y,R]...\n"
                  "               [--circles N [--seed S] [--r-min R] [--r-max R]] [--precision f32|f16s]\n"
                  "               [--boundary reference|zero-gradient] [--frames F] [--steps-per-frame K]\n"
-                 "               [--variant V] [--dump-f PATH]\n");
+                 "               [--barrier curl|chaos|welcome] [--variant V] [--dump-f PATH]\n");
 }
 
 bool parse(int argc, char **argv, Args &a) {
@@ -76,7 +79,11 @@ bool parse(int argc, char **argv, Args &a) {
         else if (k == "--steps-per-frame") a.steps_per_frame = (uint32_t)std::strtoul(val(), nullptr, 10);
         else if (k == "--variant") a.variant = std::atoi(val());
         else if (k == "--dump-f") a.dump = val();
-        else {
+        else if (k == "--barrier") {
+            a.barrier = val();
+            if (a.barrier != "curl" && a.barrier != "chaos" && a.barrier != "welcome")
+                throw lbm::Error(LBM_EINVAL, "--barrier is curl, chaos or welcome");
+        } else {
             std::fprintf(stderr, "unknown option %s\n", k.c_str());
             usage(stderr);
             return false;
@@ -110,6 +117,9 @@ int main(int argc, char **argv) {
         opt.boundary = a.boundary;
         lbm::LBM g(1.0f / (3.0f * a.viscosity + 0.5f), a.nx, a.ny, opt);
         g.set_mask(build_mask(a));
+        if (a.barrier == "curl") lbm::shapes::draw_shape(g, lbm::shapes::curl_barrier(a.nx, a.ny));
+        else if (a.barrier == "chaos") lbm::shapes::draw_shape(g, lbm::shapes::chaos_barrier(a.nx, a.ny));
+        else if (a.barrier == "welcome") lbm::shapes::draw_shape(g, lbm::shapes::welcome_barrier(a.nx, a.ny));
         g.custom_speed(a.speed);
         const int variant = a.variant >= 0 ? (g.set_variant(a.variant), a.variant) : g.autotune(10);
         for (uint32_t f = 0; f < a.frames; ++f) {

@@ -626,3 +626,14 @@ def test_cpp_driver_matches_python_and_oracle(tmp_path):
     rr.step(200)
     assert same_bits(f9, rr.state())
     g.close()
+    # a barrier preset drawn by the C++ geometry equals the Python one
+    r = subprocess.run([cli, "--nx", "400", "--ny", "160", "--barrier", "welcome", "--frames", "1",
+                        "--steps-per-frame", "60", "--variant", "8", "--dump-f", str(dump)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    g = LBM(float(omega), X1, Y1)
+    g.welcome_barrier()
+    g.custom_speed(U1)
+    g.step(60)
+    assert same_bits(np.fromfile(dump, np.float32).reshape(9, Y1, X1), g.read_f())
+    g.close()

@@ -279,3 +279,57 @@ def test_clear_barrier_forgets_history():
     ch.handle_click((9, 9))
     ch.clear_barrier()
     assert ch.undo_stack == [] and ch.history == {} and ch.current_blob.is_empty()
+
+
+# ---- the C++ host side's geometry (include/lbm_shapes.hpp) equals this module's --------
+
+def _cpp_dump(tmp_path, args):
+    import os
+    import subprocess
+    root = os.path.join(os.path.dirname(__file__), "..")
+    exe = tmp_path / "shapes_dump"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "cpp", "shapes_dump.cpp"), "-o", str(exe)], check=True, timeout=120)
+    out = subprocess.run([str(exe)] + [str(a) for a in args], check=True, capture_output=True, text=True,
+                         timeout=60).stdout
+    cases, cur = [], []
+    for line in out.splitlines():
+        if line == "--":
+            cases.append(cur)
+            cur = []
+        else:
+            cur.append(line)
+    return cases
+
+
+def test_cpp_geometry_equals_python(tmp_path):
+    rng = random.Random(5)
+    args, expect = [], []
+    for _ in range(60):
+        nx, ny = rng.randint(10, 90), rng.randint(10, 70)
+        p1 = (rng.randrange(nx), rng.randrange(ny))
+        p2 = (rng.randrange(nx), rng.randrange(ny))
+        erased = rng.random() < 0.3
+        args += ["line", *p1, *p2, nx, ny, int(erased)]
+        expect.append(S.Line(p1, p2, nx, ny, erased=erased).points)
+    args += ["line", 0, 0, 10, 5, 10, 10, 0]
+    expect.append(None)  # endpoint outside: rejected by both
+    for name, fn in (("curl", S.curl_barrier), ("chaos", S.chaos_barrier), ("welcome", S.welcome_barrier)):
+        for nx, ny in ((400, 160), (256, 164), (130, 90)):
+            args += ["preset", name, nx, ny]
+            expect.append(fn(nx, ny).points)
+    for _ in range(40):
+        a = (rng.randint(-15, 15), rng.randint(-15, 15))
+        b = (rng.randint(-15, 15), rng.randint(-15, 15))
+        args += ["bresenham", *a, *b]
+        expect.append(list(S.bresenham(a, b)))
+    got = _cpp_dump(tmp_path, args)
+    assert len(got) == len(expect)
+    for g, e in zip(got, expect):
+        if e is None:
+            assert g == ["invalid"]
+        elif isinstance(e, list):
+            assert [tuple(int(v) for v in line.split()) for line in g] == e
+        else:
+            pts = {(int(x), int(y), v == "1") for x, y, v in (line.split() for line in g)}
+            assert pts == e
