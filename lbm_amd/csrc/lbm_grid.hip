@@ -81,9 +81,9 @@ __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, type
 // columns.  Walking down the strip, phase 1 computes C_{n+1} = collide(S_{n+1})
 // of the next row, including the halo, into a ring of four rows in LDS, and
 // phase 2 pulls S_{n+2} of the current row from the ring and stores
-// collide(S_{n+2}) to A2.  C_{n+1} goes to B2 where the next step reads it
-// (cells the stream does not write) and, in the last pair of a call
-// (FINAL), everywhere.  The outlet column's wrapped pulls read column 0, which
+// collide(S_{n+2}) to A2.  C_{n+1} goes to B2 only where the next step reads
+// it (cells the stream does not write); a call therefore ends with one-step
+// launches, which leave both sets whole.  The outlet column's wrapped pulls read column 0, which
 // belongs to another strip; its C_{n+1} (column 0 is never streamed) comes
 // from k_pair_col0, launched first.  Same per-cell operations as k_cell, so
 // the same bits.
@@ -107,18 +107,17 @@ __device__ __forceinline__ void lds_barrier() {
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <class St, int MODE, bool FINAL>
+template <class St, int MODE>
 __global__ __launch_bounds__(kBlock) void k_pair(const typename St::T *A, const typename St::T *B,
                                                  typename St::T *__restrict__ A2, typename St::T *__restrict__ B2,
                                                  const typename St::T *__restrict__ col0,
                                                  const uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask,
-                                                 Geom g, float *__restrict__ mom, int nstrips, int rows, int nblocks,
-                                                 int remap) {
+                                                 Geom g, int nstrips, int rows, int nblocks, int remap) {
     using T = typename St::T;
     // C_{n+1} of four rows: the eight moving populations (the rest population
     // is only read by its own cell, which keeps it in a register)
     __shared__ T ring[kRing][8][kBlock];
-    const long long ks = g.ks, ps = g.pstride;
+    const long long ks = g.ks;
     const int L = remap ? xcd_remap(blockIdx.x, nblocks) : (int)blockIdx.x;
     const int chunk = L / nstrips;
     const int t = (int)threadIdx.x;
@@ -180,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(const typename St::T *A, const 
             }
         }
         St::collide(f, g.omega);
-        const bool store = owner && r >= r0 && r < r1 && (FINAL || !streamed);
+        const bool store = owner && r >= r0 && r < r1 && !streamed;
         T o[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
@@ -231,21 +230,14 @@ __global__ __launch_bounds__(kBlock) void k_pair(const typename St::T *A, const 
                 }
             }
         }
-        const Moments m = St::collide(f, g.omega);
+        St::collide(f, g.omega);
         T *dst = A2 + (size_t)y * g.ys;
 #pragma unroll
         for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(St::store(f[k]), dst + (size_t)k * ks + x);
-        if (FINAL) {
-            const size_t own = (size_t)y * g.pitch + x;
-            mom[own] = m.jx;
-            mom[ps + own] = m.jy;
-            mom[2 * ps + own] = m.rho;
-        }
     };
 
     // rows r0 - 1 and r0, then per row y: row y + 2's pulls issued, row y + 1
-    // finished into the ring, barrier, row y's second step; unrolled by two so
-    // the two register sets alternate instead of being copied
+    // finished into the ring, barrier, row y's second step
     T v[9];
     uint32_t fl = 0;
     load1(r0 - 1, v, fl);
@@ -517,7 +509,7 @@ struct lbm_grid {
     int pair_rows() const { return 8 << (variant - kPairFirst); }
 
     template <class St, int MODE>
-    void launch_pair(bool fin) {
+    void launch_pair() {
         using T = typename St::T;
         if (!spare[0]) {
             spare[0] = dalloc<char>(set_bytes());
@@ -537,12 +529,8 @@ struct lbm_grid {
         const int rows = pair_rows();
         const int nstrips = ((int)X + kBlock - 3) / (kBlock - 2);
         const int nblocks = nstrips * (((int)Y + rows - 1) / rows);
-        if (fin)
-            hipLaunchKernelGGL((k_pair<St, MODE, true>), dim3(nblocks), dim3(kBlock), 0, stream, A, B, A2, B2,
-                               (const T *)col0, flags, mask, geom, lag, nstrips, rows, nblocks, 1);
-        else
-            hipLaunchKernelGGL((k_pair<St, MODE, false>), dim3(nblocks), dim3(kBlock), 0, stream, A, B, A2, B2,
-                               (const T *)col0, flags, mask, geom, lag, nstrips, rows, nblocks, 1);
+        hipLaunchKernelGGL((k_pair<St, MODE>), dim3(nblocks), dim3(kBlock), 0, stream, A, B, A2, B2,
+                           (const T *)col0, flags, mask, geom, nstrips, rows, nblocks, 1);
         HIPCHK(hipGetLastError());
         spare[0] = f[cur];
         spare[1] = f[1 - cur];
@@ -561,15 +549,15 @@ struct lbm_grid {
         col0 = nullptr;
     }
 
-    // two P-form steps; `fin`: the last launch of a step call (writes every
-    // cell of the other set and the renderer's moments)
-    void pair(bool fin) {
+    // two P-form steps in one launch; the other set is then whole only on the
+    // cells the stream does not write
+    void pair() {
         if (precision == LBM_PRECISION_F32) {
-            if (boundary == LBM_BOUNDARY_REFERENCE) launch_pair<StoreF32, LBM_BOUNDARY_REFERENCE>(fin);
-            else launch_pair<StoreF32, LBM_BOUNDARY_ZERO_GRADIENT>(fin);
+            if (boundary == LBM_BOUNDARY_REFERENCE) launch_pair<StoreF32, LBM_BOUNDARY_REFERENCE>();
+            else launch_pair<StoreF32, LBM_BOUNDARY_ZERO_GRADIENT>();
         } else {
-            if (boundary == LBM_BOUNDARY_REFERENCE) launch_pair<StoreF16S, LBM_BOUNDARY_REFERENCE>(fin);
-            else launch_pair<StoreF16S, LBM_BOUNDARY_ZERO_GRADIENT>(fin);
+            if (boundary == LBM_BOUNDARY_REFERENCE) launch_pair<StoreF16S, LBM_BOUNDARY_REFERENCE>();
+            else launch_pair<StoreF16S, LBM_BOUNDARY_ZERO_GRADIENT>();
         }
     }
 
@@ -637,10 +625,12 @@ struct lbm_grid {
     void step(uint32_t n) {
         for (uint32_t i = 0; i < n; ++i) {
             const bool last = i + 1 == n;
-            if (pform && pair_variant(variant) && n - i >= 2) {
-                // between launches of one call the other set is current only on
-                // cells the stream does not write, the only ones read from it
-                pair(i + 2 == n);
+            if (pform && pair_variant(variant) && n - i >= 3) {
+                // after a two-step launch the other set is current only on the
+                // cells the stream does not write, the only ones the next step
+                // reads from it; the last step of a call is always a one-step
+                // launch, which writes that set whole (and the moments)
+                pair();
                 ++i;
                 continue;
             }
