@@ -66,7 +66,7 @@ def run_gpu(precision, X, Y, mask, steps, warmup, omega, u_in, variant=None):
         g.variant = variant
         tuned = {"variant": variant, "us_per_step_by_variant": None}
     else:
-        best, ms = g.autotune(20)  # state is restored afterwards
+        best, ms = g.autotune(100)  # state is restored afterwards; 100 steps keep the pick stable
         tuned = {"variant": best, "us_per_step_by_variant": [round(m * 1e3, 2) for m in ms]}
         log(f"autotune: variant {best}, us/step {tuned['us_per_step_by_variant']}")
     g.step(warmup)
