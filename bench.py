@@ -113,7 +113,7 @@ def surrogate_leg(steps=10000):
         g.mask = mask
         g.custom_speed(u)
         g.step(2000)
-        g.autotune(20)  # the fastest step kernel for both sides (state restored)
+        g.autotune(60)  # the fastest step kernel for both sides (state restored)
         return g
     net = UNet(1, Y // 2, X // 2, seed=1)
     params = load_params(SURROGATE_WEIGHTS)

@@ -121,7 +121,7 @@ int main(int argc, char **argv) {
         else if (a.barrier == "chaos") lbm::shapes::draw_shape(g, lbm::shapes::chaos_barrier(a.nx, a.ny));
         else if (a.barrier == "welcome") lbm::shapes::draw_shape(g, lbm::shapes::welcome_barrier(a.nx, a.ny));
         g.custom_speed(a.speed);
-        const int variant = a.variant >= 0 ? (g.set_variant(a.variant), a.variant) : g.autotune(10);
+        const int variant = a.variant >= 0 ? (g.set_variant(a.variant), a.variant) : g.autotune(60);
         for (uint32_t f = 0; f < a.frames; ++f) {
             g.sync();
             const auto t0 = std::chrono::steady_clock::now();
