@@ -1,5 +1,6 @@
-/* Obstacle geometry for the C++ host side (lbm-wgpu src/barrier_shapes/*.rs and
- * the barrier presets of src/lbm.rs:1367-1480): Bresenham lines with diagonal
+/* Obstacle geometry for the C++ host side (lbm-wgpu src/barrier_shapes/ line.rs,
+ * blob.rs, curve.rs, curve_collection.rs and merge_shapes.rs, and the barrier
+ * presets of src/lbm.rs:1367-1480): Bresenham lines with diagonal
  * filling, curves, blobs, merging, get_points_vector's [x + y * nx, 1|0, ...]
  * wire format for lbm_paint, and the curl / chaos / welcome presets.
  *

@@ -637,3 +637,20 @@ def test_cpp_driver_matches_python_and_oracle(tmp_path):
     g.step(60)
     assert same_bits(np.fromfile(dump, np.float32).reshape(9, Y1, X1), g.read_f())
     g.close()
+
+
+def test_cpp_mirror_api(tmp_path):
+    """include/lbm_hip.hpp on the GPU: the reference's surface, summary and colour
+    planes, every variant continuing one state bit for bit, and errors as
+    lbm::Error with the C ABI's code (tests/cpp/api_check.cpp)."""
+    import os
+    import subprocess
+    print("case: C++ mirror API", flush=True)
+    root = os.path.join(os.path.dirname(__file__), "..")
+    exe = tmp_path / "api_check"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "cpp", "api_check.cpp"), "-L", os.path.join(root, "lbm_amd"),
+                    "-l:liblbm_hip.so", "-Wl,-rpath," + os.path.abspath(os.path.join(root, "lbm_amd")), "-o", str(exe)],
+                   check=True, timeout=180)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
