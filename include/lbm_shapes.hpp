@@ -196,6 +196,17 @@ struct Curve : Shape {
     }
 };
 
+/* the finished strokes of one drag (curve_collection.rs:4-29) */
+struct CurveCollection : Shape {
+    bool has_last = false;
+    XY last{0, 0};
+    void add_curve(const Curve &c) {
+        has_last = c.has_last;
+        last = c.last;
+        points.insert(c.points.begin(), c.points.end());
+    }
+};
+
 /* get_points_vector (merge_shapes.rs:12-22), points in row-major order */
 inline std::vector<uint32_t> points_vector(const Shape &s, uint32_t xdim) {
     std::vector<Point> pts(s.points.begin(), s.points.end());
