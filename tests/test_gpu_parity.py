@@ -491,15 +491,17 @@ def test_autotune_leaves_the_state_untouched():
     assert same_bits(a.moments(True)[0], b.moments(True)[0])
 
 
+@pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("cmap", [0, 1, 2])
-def test_render_colours_equal_colormap_restatement(cmap):
+def test_render_colours_equal_colormap_restatement(cmap, precision):
     """lbm_render: the summary plane mapped to colours in the same pass equals
-    the CPU restatement of color_map/*.wgsl bit for bit, for every summary."""
-    print(f"case: render colormap {cmap}", flush=True)
+    the CPU restatement of color_map/*.wgsl bit for bit, for every summary and
+    both storage formats."""
+    print(f"case: render colormap {cmap} precision {precision}", flush=True)
     from lbm_amd import LBM
     from oracle import colormap_ref
     mask = circle_mask(X1, Y1, [(80, 80, 16)])
-    g = LBM(OMEGA1, X1, Y1)
+    g = LBM(OMEGA1, X1, Y1, precision=precision)
     g.mask = mask
     g.custom_speed(U1)
     g.step(300)
