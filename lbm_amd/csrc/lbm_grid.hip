@@ -19,6 +19,8 @@
 //   fused   (P -> P)   S_n := pull(buf[cur]) on streamed cells, buf[1-cur] own
 //                      values elsewhere; buf[1-cur] := collide(S_n); swap.
 //   flush   (P -> S)   buf[1-cur] := S_n (streamed cells only); swap.
+//   pair    (P -> P -> P, variants 8 and 9) two fused steps in one pass, the
+//                      first one's result staged in LDS (k_pair below).
 // Since the pull of step n uses the mask at the time the kernel runs, any mask
 // change first flushes a pending stream, so results are those of the reference
 // step order exactly.  The rest population is stored in both sets (the
