@@ -141,11 +141,14 @@ int lbm_sync(lbm_grid *g);
 /* Step-kernel variants (no reference counterpart): how the fused step moves
  * memory -- cells per thread, non-temporal access, block order, or (8, 9) two
  * steps per launch through LDS, which reads and writes the state once per two
- * steps (and allocates two more state sets on its first launch, freed again
- * when another variant is set or chosen).  All variants compute the same bits.  lbm_autotune times each for
- * `steps` steps on the grid's own state, restores that state exactly, selects
- * the fastest, and returns it in *best and the per-step time of every variant
- * in ms[LBM_NUM_VARIANTS] (either pointer may be NULL). */
+ * steps.  All variants compute the same bits.  The two-step variants need two
+ * more state sets: lbm_set_variant allocates them when it selects one (and
+ * fails, changing nothing, when it cannot) and frees them when it selects
+ * another.  lbm_autotune times each variant for `steps` steps on the grid's
+ * own state, restores that state exactly, selects the fastest, and returns it
+ * in *best and the per-step time of every variant in ms[LBM_NUM_VARIANTS]
+ * (either pointer may be NULL; +inf for a two-step variant whose sets could
+ * not be allocated). */
 #define LBM_NUM_VARIANTS 10
 int lbm_set_variant(lbm_grid *g, int variant);
 int lbm_get_variant(const lbm_grid *g, int *variant);

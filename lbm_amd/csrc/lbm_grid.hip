@@ -510,17 +510,28 @@ struct lbm_grid {
     static bool pair_variant(int v) { return v >= kPairFirst; }
     int pair_rows() const { return 8 << (variant - kPairFirst); }
 
-    template <class St, int MODE>
-    void launch_pair() {
-        using T = typename St::T;
-        if (!spare[0]) {
+    // the two-step kernel's output sets and column buffer; allocated when a
+    // two-step variant is set, so that stepping itself does not allocate
+    void ensure_pair_buffers() {
+        if (spare[0] && spare[1] && col0) return;
+        release_pair_buffers();
+        try {
             spare[0] = dalloc<char>(set_bytes());
             spare[1] = dalloc<char>(set_bytes());
             col0 = dalloc<char>(9 * (size_t)Y * esize);
             // the rows' padding is never written by this kernel: keep it defined
             HIPCHK(hipMemsetAsync(spare[0], 0, set_bytes(), stream));
             HIPCHK(hipMemsetAsync(spare[1], 0, set_bytes(), stream));
+        } catch (...) {
+            release_pair_buffers();
+            throw;
         }
+    }
+
+    template <class St, int MODE>
+    void launch_pair() {
+        using T = typename St::T;
+        ensure_pair_buffers();
         const T *A = (const T *)f[cur], *B = (const T *)f[1 - cur];
         T *A2 = (T *)spare[0], *B2 = (T *)spare[1];
         if (MODE == LBM_BOUNDARY_REFERENCE) {
@@ -1019,8 +1030,9 @@ int lbm_set_variant(lbm_grid *g, int variant) {
     return guarded(g, [&] {
         check(variant >= 0 && variant < LBM_NUM_VARIANTS, "unknown kernel variant");
         HIPCHK(hipStreamSynchronize(g->stream));
+        if (g->pair_variant(variant)) g->ensure_pair_buffers();  // may fail: then nothing changes
+        else g->release_pair_buffers();
         g->variant = variant;
-        if (!g->pair_variant(variant)) g->release_pair_buffers();
     });
 }
 
@@ -1049,6 +1061,14 @@ int lbm_autotune(lbm_grid *g, uint32_t steps, int *best, float *ms) {
             HIPCHK(hipEventCreate(&e0));
             HIPCHK(hipEventCreate(&e1));
             for (int v = 0; v < LBM_NUM_VARIANTS; ++v) {
+                if (g->pair_variant(v)) {
+                    try {
+                        g->ensure_pair_buffers();
+                    } catch (...) {  // no memory for two more sets: not a candidate
+                        t[v] = INFINITY;
+                        continue;
+                    }
+                }
                 g->variant = v;
                 g->step(2);  // warm-up (and leaves the grid in P-form)
                 HIPCHK(hipEventRecord(e0, g->stream));
