@@ -656,3 +656,26 @@ def test_cpp_mirror_api(tmp_path):
                    check=True, timeout=180)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
+
+
+def test_run_script_mode(tmp_path, capsys):
+    """python -m lbm_amd.run --script: the control commands from a file, one
+    output line per rendered frame; a bad command stops with exit code 2."""
+    print("case: run --script", flush=True)
+    from lbm_amd import run
+    script = tmp_path / "controls.txt"
+    script.write_text("# preset, a stroke, undo, then step mode\n"
+                      "preset chaos\nframe\n"
+                      "press 10 10\nmove 30 20\nrelease\nframe\n"
+                      "undo\nframe\n"
+                      "step-mode on\nstep\nframe\n"
+                      f"save {tmp_path / 'last.ppm'}\n")
+    rc = run.main(["--nx", "128", "--ny", "64", "--script", str(script)])
+    out = [line for line in capsys.readouterr().out.splitlines() if line.startswith("line ")]
+    assert rc == 0
+    assert [line.split(":")[0] for line in out] == ["line 3", "line 7", "line 9", "line 12"]
+    assert out[-1].startswith("line 12: step 46 ")  # 3 x 15 steps, then one in step mode
+    assert (tmp_path / "last.ppm").exists()
+    bad = tmp_path / "bad.txt"
+    bad.write_text("frame\nexplode\n")
+    assert run.main(["--nx", "64", "--ny", "32", "--script", str(bad)]) == 2
