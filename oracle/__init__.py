@@ -16,7 +16,8 @@ Contents
 Pins (tests/test_oracle.py): omega = 0 pure-streaming permutation and the
 bounce-back known-answer tables, equilibrium identities and the f32 fixed point,
 closed-box mass conservation, per-cell collision conservation, RefCPU == np_ref
-bit for bit, RefCPU vs Oracle64 at the f32 rounding level.  The reference has no
+bit for bit, RefCPU vs Oracle64 at the f32 rounding level, and the summary pass
+(curl, display density and speed, velocity planes) against closed forms.  The reference has no
 tests, fixtures or golden vectors of its own for this path.
 """
 from __future__ import annotations

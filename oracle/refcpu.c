@@ -32,7 +32,10 @@
  * races for duplicates).
  *
  * Parity pins: see tests/test_oracle.py (closed forms, permutation and
- * bounce-back known-answer tables, conservation, a second numpy restatement).
+ * bounce-back known-answer tables, conservation, a second numpy restatement;
+ * the summary pass against the closed-form curl of linear momentum fields, the
+ * flattened-index east neighbour of the last column, the skip rule and both
+ * clamp edges of the display density and speed).
  */
 #include <stdint.h>
 #include <stdlib.h>

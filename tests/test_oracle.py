@@ -307,3 +307,125 @@ def test_colormap_is_continuous_and_clamped(which):
     mid = cm.colormap(which, np.float32(0.5 / s))        # halfway through block 0
     assert np.allclose(mid, 0.5 * (tab[n] + tab[n + 1]), atol=1e-6)
     assert not cm.colormap(which, np.array([0.01, 0.2], np.float32), np.array([1, 1])).any()
+
+
+# ---- summary pass (refcpu.c ref_summary, summary_stats/*.wgsl) ----------------------
+
+def _state_with_moments(jx, jy, rho):
+    """Populations whose reference moments (pre_collision/*.wgsl) are exactly
+    (jx, jy, rho): E - W = jx, N - S = jy (north-positive), every other moving
+    population 0 and the rest population C = rho - E - W - N - S.  The inputs
+    are dyadic with few bits, so every sum the shaders form is exact in f32."""
+    Y, X = jx.shape
+    f = np.zeros((9, Y, X), np.float64)
+    f[5] = np.maximum(jx, 0)   # E
+    f[3] = np.maximum(-jx, 0)  # W
+    f[1] = np.maximum(jy, 0)   # N
+    f[7] = np.maximum(-jy, 0)  # S
+    f[4] = rho - f[5] - f[3] - f[1] - f[7]
+    assert np.array_equal(f.astype(np.float32).astype(np.float64), f)
+    return f.astype(np.float32)
+
+
+def _fields_oracle(jx, jy, rho):
+    """refcpu with density planes (jx, jy, rho): set 0 holds populations with
+    those moments and one collide pass (step parity 0) computes them, rest
+    population included (corner_collision.wgsl:24), into the density planes."""
+    Y, X = jx.shape
+    r = oracle.RefCPU(X, Y, 1.0)
+    f = _state_with_moments(jx, jy, rho)
+    r.set_set(0, f)
+    r.set_set(1, f)
+    r.collide()
+    djx, djy, drho = r.density()
+    # the moments the pass extracted are the prescribed ones, bit for bit
+    assert np.array_equal(djx, jx.astype(np.float32))
+    assert np.array_equal(djy, jy.astype(np.float32))
+    assert np.array_equal(drho, rho.astype(np.float32))
+    return r
+
+
+def _vorticity_curl(a, b, Omega, rho):
+    """Closed form of the reference's curl for the momentum field
+    j = (a row + Omega row, b x + Omega x) (rows grow southwards, y north-positive
+    = -row): curl.wgsl:49 is 10 * (central differences over 2 cells) / rho, i.e.
+    20 (d jy/dx - d jx/dy) / rho, exact for a linear field:
+    d jy/dx = b + Omega, d jx/dy = -(a + Omega)."""
+    return 20.0 * ((b + Omega) + (a + Omega)) / rho
+
+
+@pytest.mark.parametrize("a,b,Om", [(1 / 64, 0.0, 0.0), (0.0, 1 / 32, 0.0), (1 / 64, 1 / 32, 0.0),
+                                    (0.0, 0.0, 1 / 128), (-3 / 64, 1 / 16, 0.0)])
+def test_summary_curl_of_linear_momentum_fields(oracle_lib, a, b, Om):
+    """Linear shear (jx = a row), a jy = b x field and solid-body rotation
+    (jx = Om row, jy = Om x): the curl equals its closed form on every cell the
+    pass writes, except the last column, whose east neighbour is the next row's
+    column 0 (flattened index, curl.wgsl:49): there jy[i + 1] = b * 0 = 0.
+    Where the pass returns early (x = 0, curl.wgsl:41; the last row,
+    curl.wgsl:45) the output plane keeps what the previous pass wrote."""
+    X, Y = 13, 9
+    yy, xx = np.mgrid[0:Y, 0:X].astype(np.float64)
+    jx = (a + Om) * yy
+    jy = (b + Om) * xx
+    rho = 1.0 + 0.25 * (xx % 3)  # 1, 1.25, 1.5
+    r = _fields_oracle(jx, jy, rho)
+    disp = r.summary(3)  # fills the output plane with the display density first
+    curl = r.summary(0).astype(np.float64)
+    inner = (slice(1, Y - 1), slice(1, X - 1))
+    np.testing.assert_allclose(curl[inner], _vorticity_curl(a, b, Om, rho)[inner], rtol=2 ** -22, atol=0)
+    # last column: the east neighbour is (0, row + 1), whose jy is 0
+    xl = X - 1
+    rows = np.arange(1, Y - 1)
+    exp_last = 10.0 * (0.0 - (b + Om) * (xl - 1) + 2 * (a + Om)) / rho[rows, xl]
+    np.testing.assert_allclose(curl[rows, xl], exp_last, rtol=2 ** -22, atol=1e-12)
+    # skipped cells keep the previous pass's values
+    assert np.array_equal(curl[:, 0], disp[:, 0].astype(np.float64))
+    assert np.array_equal(curl[Y - 1, :], disp[Y - 1, :].astype(np.float64))
+
+
+def test_summary_curl_sign_convention_against_a_vortex(oracle_lib):
+    """A counter-clockwise vortex seen with y north-up (rows grow downwards)
+    has positive vorticity: j = Om (-(y - yc), x - xc) with y = -row gives
+    jx = Om (row - rc), jy = Om (x - xc), and a positive curl everywhere."""
+    X, Y, Om = 11, 11, 1 / 64
+    yy, xx = np.mgrid[0:Y, 0:X].astype(np.float64)
+    jx, jy = Om * (yy - 5), Om * (xx - 5)
+    r = _fields_oracle(jx, jy, np.ones((Y, X)))
+    curl = r.summary(0)
+    assert np.all(curl[1:-1, 1:-1] == np.float32(40 * Om))
+
+
+@pytest.mark.parametrize("rho,expect", [(-1.0, -0.5), (0.0, -0.5), (1.0, 0.1), (2.0, 0.7), (6.0, 3.1),
+                                        (7.0, 3.5), (10.0, 3.5)])
+def test_summary_display_density(oracle_lib, rho, expect):
+    """rho.wgsl:24: 4 clamp(0.15 rho, 0, 1) - 0.5 on every cell, both clamp
+    edges included (0.15 rho < 0 for rho < 0; > 1 for rho > 6.67)."""
+    X, Y = 8, 5
+    r = _fields_oracle(np.zeros((Y, X)), np.zeros((Y, X)), np.full((Y, X), rho))
+    out = r.summary(3)
+    np.testing.assert_allclose(out, expect, rtol=0, atol=4e-7)
+
+
+@pytest.mark.parametrize("jx,jy,expect", [(0.0, 0.0, -0.5), (0.0625, 0.0, -0.1875), (0.0, -0.125, 0.125),
+                                          (0.0625, 0.125, None), (0.25, 0.25, 0.5), (-1.5, 2.0, 0.5)])
+def test_summary_display_speed(oracle_lib, jx, jy, expect):
+    """speed.wgsl:25: clamp(5 |j|, 0, 1) - 0.5 of the momentum (not the velocity:
+    rho does not enter), clamped above at |j| = 0.2."""
+    X, Y = 8, 5
+    rho = np.full((Y, X), 2.0)  # a non-unit density: a division by rho would show
+    r = _fields_oracle(np.full((Y, X), jx), np.full((Y, X), jy), rho)
+    out = r.summary(4).astype(np.float64)
+    if expect is None:
+        expect = min(5 * np.hypot(jx, jy), 1.0) - 0.5
+    np.testing.assert_allclose(out, expect, rtol=0, atol=2e-7)
+
+
+def test_summary_velocity_planes_are_momenta(oracle_lib):
+    """ux.wgsl:41 / uy.wgsl:41 copy the density bind group's jx, jy (momenta,
+    jy north-positive) unchanged."""
+    X, Y = 9, 6
+    yy, xx = np.mgrid[0:Y, 0:X].astype(np.float64)
+    jx, jy = (xx - 4) / 32, (3 - yy) / 16
+    r = _fields_oracle(jx, jy, 2.0 + 0 * xx)
+    assert np.array_equal(r.summary(1), jx.astype(np.float32))
+    assert np.array_equal(r.summary(2), jy.astype(np.float32))
