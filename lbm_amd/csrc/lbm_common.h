@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <cstdlib>
 #include <string>
 
 #include "../../include/lbm_hip.h"
@@ -659,22 +661,43 @@ struct HipError {
     const char *what;
 };
 
+// A failed runtime call is reported by the exception; its status is also
+// cleared from the thread's last error so that a later launch check does not
+// report it a second time.
 #define HIPCHK(expr)                                  \
     do {                                              \
         hipError_t e_ = (expr);                       \
-        if (e_ != hipSuccess) throw HipError{e_, #expr}; \
+        if (e_ != hipSuccess) {                       \
+            (void)hipGetLastError();                  \
+            throw HipError{e_, #expr};                \
+        }                                             \
     } while (0)
 
 struct ArgError {
     std::string msg;
 };
 
+// Test hook: with LBM_TEST_ALLOC_LIMIT=<bytes> in the environment, a request
+// above that many bytes is passed on as an impossible size, so the runtime
+// fails it the way it fails a real out-of-memory request.
+inline size_t test_alloc_limit() {
+    const char *s = std::getenv("LBM_TEST_ALLOC_LIMIT");
+    return s && *s ? (size_t)std::strtoull(s, nullptr, 10) : SIZE_MAX;
+}
+
 template <class T>
 T *dalloc(size_t n) {
     void *p = nullptr;
     if (n == 0) n = 1;
-    hipError_t e = hipMalloc(&p, n * sizeof(T));
-    if (e != hipSuccess) throw HipError{e, "hipMalloc"};
+    const size_t bytes = n * sizeof(T);
+    hipError_t e = hipMalloc(&p, bytes > test_alloc_limit() ? (size_t)1 << 62 : bytes);
+    if (e != hipSuccess) {
+        // the runtime keeps a failed call's status as the thread's last error,
+        // which the next launch check would report: clear it, the failure is
+        // reported here
+        (void)hipGetLastError();
+        throw HipError{e, "hipMalloc"};
+    }
     return (T *)p;
 }
 

@@ -125,7 +125,10 @@ struct AErr {
 #define H64(x)                                  \
     do {                                        \
         hipError_t e_ = (x);                    \
-        if (e_ != hipSuccess) throw HErr{e_, #x}; \
+        if (e_ != hipSuccess) {                 \
+            (void)hipGetLastError();            \
+            throw HErr{e_, #x};                 \
+        }                                       \
     } while (0)
 
 thread_local std::string g_f64_error;

@@ -385,6 +385,8 @@ uint16_t *unet_input_tensor(lbm_unet *n);
 float *unet_output_tensor(lbm_unet *n);
 void unet_shape(const lbm_unet *n, int *b, int *h, int *w);
 void unet_forward_on(lbm_unet *n, hipStream_t s);
+void unet_begin_use(lbm_unet *n, hipStream_t s);
+void unet_end_use(lbm_unet *n, hipStream_t s);
 int unet_device(const lbm_unet *n);
 
 struct lbm_grid {
@@ -1051,16 +1053,25 @@ int lbm_autotune(lbm_grid *g, uint32_t steps, int *best, float *ms) {
         const int cur = g->cur;
         const bool pform = g->pform;
         const uint64_t nsteps = g->steps;
+        const int variant = g->variant;
         float t[LBM_NUM_VARIANTS];
         int v_best = 0;
+        bool saved = false;
         try {
             HIPCHK(hipMemcpyAsync(snap, g->f[0], sb, hipMemcpyDeviceToDevice, g->stream));
             HIPCHK(hipMemcpyAsync(snap + sb, g->f[1], sb, hipMemcpyDeviceToDevice, g->stream));
             HIPCHK(hipMemcpyAsync(snap + 2 * sb, g->lag, lb, hipMemcpyDeviceToDevice, g->stream));
+            saved = true;
             hipEvent_t e0, e1;
             HIPCHK(hipEventCreate(&e0));
             HIPCHK(hipEventCreate(&e1));
             for (int v = 0; v < LBM_NUM_VARIANTS; ++v) {
+                if (g->pair_variant(v) && steps < 3) {
+                    // a call of fewer than three steps never launches the
+                    // two-step kernel: it would time the one-step fallback
+                    t[v] = INFINITY;
+                    continue;
+                }
                 if (g->pair_variant(v)) {
                     try {
                         g->ensure_pair_buffers();
@@ -1085,8 +1096,22 @@ int lbm_autotune(lbm_grid *g, uint32_t steps, int *best, float *ms) {
             HIPCHK(hipMemcpyAsync(g->lag, snap + 2 * sb, lb, hipMemcpyDeviceToDevice, g->stream));
             HIPCHK(hipStreamSynchronize(g->stream));
         } catch (...) {
+            // put the grid back as it was (the two-step kernel trades the
+            // state sets with its spare pair, so the state may live in any
+            // two of the four: the copies restore whichever two are current)
             (void)hipStreamSynchronize(g->stream);
+            if (saved) {
+                (void)hipMemcpyAsync(g->f[0], snap, sb, hipMemcpyDeviceToDevice, g->stream);
+                (void)hipMemcpyAsync(g->f[1], snap + sb, sb, hipMemcpyDeviceToDevice, g->stream);
+                (void)hipMemcpyAsync(g->lag, snap + 2 * sb, lb, hipMemcpyDeviceToDevice, g->stream);
+                (void)hipStreamSynchronize(g->stream);
+            }
             (void)hipFree(snap);
+            g->cur = cur;
+            g->pform = pform;
+            g->steps = nsteps;
+            g->variant = variant;
+            if (!g->pair_variant(variant)) g->release_pair_buffers();
             throw;
         }
         (void)hipFree(snap);
@@ -1175,6 +1200,7 @@ int lbm_surrogate_jump(lbm_grid *g, lbm_unet *net, uint32_t steps, float s_rho, 
         check(unet_device(net) == g->dev, "network and grid on different devices");
         g->flush();  // S-form: buf[cur] is the state to predict from and to overwrite
         g->moments_now(false);
+        unet_begin_use(net, g->stream);  // after any use of the network on another stream
         const dim3 pgrid((w + 255) / 256, h);
         if (fine)
             hipLaunchKernelGGL(k_pack_net_input<1>, pgrid, dim3(256), 0, g->stream, g->now, g->mask,
@@ -1198,6 +1224,7 @@ int lbm_surrogate_jump(lbm_grid *g, lbm_unet *net, uint32_t steps, float s_rho, 
         }
 #undef APPLY
         HIPCHK(hipGetLastError());
+        unet_end_use(net, g->stream);
         g->steps += steps;
     });
 }
@@ -1328,7 +1355,10 @@ struct StateHeader {
     uint32_t pitch;
     long long pstride;
     uint64_t steps;
+    float u_in;  // the inflow speed of the last uniform reset (the surrogate's input)
+    uint32_t reserved;
 };
+constexpr char kStateMagic[9] = "LBMHIP02";
 }  // namespace
 
 int lbm_state_size(const lbm_grid *g, size_t *bytes) {
@@ -1344,7 +1374,7 @@ int lbm_save_state(lbm_grid *g, void *buf, size_t bytes) {
         lbm_state_size(g, &need);
         check(buf != nullptr && bytes >= need, "buffer too small");
         StateHeader h{};
-        std::memcpy(h.magic, "LBMHIP01", 8);
+        std::memcpy(h.magic, kStateMagic, 8);
         h.X = g->X;
         h.Y = g->Y;
         h.precision = g->precision;
@@ -1355,6 +1385,7 @@ int lbm_save_state(lbm_grid *g, void *buf, size_t bytes) {
         h.pitch = (uint32_t)g->geom.pitch;
         h.pstride = g->geom.pstride;
         h.steps = g->steps;
+        h.u_in = g->u_in;
         char *p = (char *)buf;
         std::memcpy(p, &h, sizeof h);
         p += sizeof h;
@@ -1379,10 +1410,13 @@ int lbm_load_state(lbm_grid *g, const void *buf, size_t bytes) {
         check(buf != nullptr && bytes >= need, "buffer too small");
         StateHeader h;
         std::memcpy(&h, buf, sizeof h);
-        check(std::memcmp(h.magic, "LBMHIP01", 8) == 0, "not a state image");
+        check(std::memcmp(h.magic, kStateMagic, 8) == 0, "not a state image");
         check(h.X == g->X && h.Y == g->Y && h.precision == g->precision && h.boundary == g->boundary &&
                   (int)h.pitch == g->geom.pitch && h.pstride == g->geom.pstride,
               "state image does not match this grid");
+        check((h.cur == 0 || h.cur == 1) && (h.pform == 0 || h.pform == 1) && std::isfinite(h.omega) &&
+                  std::isfinite(h.u_in),
+              "corrupt state image header");
         const char *p = (const char *)buf + sizeof h;
         const size_t ps = (size_t)g->geom.pstride;
         HIPCHK(hipMemcpyAsync(g->f[0], p, g->set_bytes(), hipMemcpyHostToDevice, g->stream));
@@ -1396,6 +1430,7 @@ int lbm_load_state(lbm_grid *g, const void *buf, size_t bytes) {
         HIPCHK(hipMemcpyAsync(g->out, p, ps * sizeof(float), hipMemcpyHostToDevice, g->stream));
         g->omega = h.omega;
         g->geom.omega = h.omega;
+        g->u_in = h.u_in;
         g->cur = h.cur;
         g->pform = h.pform != 0;
         g->steps = h.steps;

@@ -781,6 +781,20 @@ struct lbm_unet {
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // the network's tensors are shared by every stream that uses it (its own and
+    // the streams of the grids that jump with it): `used` marks the end of the
+    // last use enqueued on `used_on`, and a use on another stream waits for it
+    hipEvent_t used = nullptr;
+    hipStream_t used_on = nullptr;
+
+    void order_after_last_use(hipStream_t s) {
+        if (used_on && used_on != s) HIPCHK(hipStreamWaitEvent(s, used, 0));
+    }
+    void mark_use(hipStream_t s) {
+        if (!used) HIPCHK(hipEventCreateWithFlags(&used, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(used, s));
+        used_on = s;
+    }
 
     void *alloc(size_t bytes) {
         void *p = dalloc<char>(bytes);
@@ -803,6 +817,11 @@ struct lbm_unet {
     }
     void release() {
         (void)hipSetDevice(dev);
+        if (used) {
+            (void)hipEventSynchronize(used);
+            (void)hipEventDestroy(used);
+            used = nullptr;
+        }
         drop_graph();
         for (void *p : mem) (void)hipFree(p);
         mem.clear();
@@ -1078,6 +1097,9 @@ void unet_shape(const lbm_unet *n, int *b, int *h, int *w) {
 // network's lifetime (weights are updated in place), so it is captured once as
 // a hipGraph (no memset nodes) and replayed: at surrogate sizes the ~30
 // launches cost as much as their work.
+void unet_begin_use(lbm_unet *n, hipStream_t s) { n->order_after_last_use(s); }
+void unet_end_use(lbm_unet *n, hipStream_t s) { n->mark_use(s); }
+
 void unet_forward_on(lbm_unet *n, hipStream_t s) {
     hipStream_t saved = n->stream;
     n->stream = s;
@@ -1139,6 +1161,7 @@ int lbm_unet_set_graph(lbm_unet *n, int enable) {
     return uguard([&] {
         ucheck(n != nullptr, "null network");
         HIPCHK(hipSetDevice(n->dev));
+        if (n->used) HIPCHK(hipEventSynchronize(n->used));  // a jump may still run the graph
         n->use_graph = enable != 0;
         if (!n->use_graph) n->drop_graph();
     });
@@ -1162,6 +1185,7 @@ int lbm_unet_get_params(lbm_unet *n, float *params) {
     return uguard([&] {
         ucheck(n && params, "null argument");
         HIPCHK(hipSetDevice(n->dev));
+        n->order_after_last_use(n->stream);
         HIPCHK(hipMemcpyAsync(params, n->w32, n->nparam * 4, hipMemcpyDeviceToHost, n->stream));
         HIPCHK(hipStreamSynchronize(n->stream));
     });
@@ -1171,6 +1195,7 @@ int lbm_unet_set_params(lbm_unet *n, const float *params) {
     return uguard([&] {
         ucheck(n && params, "null argument");
         HIPCHK(hipSetDevice(n->dev));
+        n->order_after_last_use(n->stream);
         HIPCHK(hipMemcpyAsync(n->w32, params, n->nparam * 4, hipMemcpyHostToDevice, n->stream));
         n->refresh_bf16();
         HIPCHK(hipStreamSynchronize(n->stream));
@@ -1181,6 +1206,7 @@ int lbm_unet_forward(lbm_unet *n, const float *x, float *y) {
     return uguard([&] {
         ucheck(n && x && y, "null argument");
         HIPCHK(hipSetDevice(n->dev));
+        n->order_after_last_use(n->stream);
         const size_t P = (size_t)n->B * n->H * n->W;
         HIPCHK(hipMemcpyAsync(n->fields, x, P * 32, hipMemcpyHostToDevice, n->stream));
         hipLaunchKernelGGL(k_pack_input, dim3(nblk(P)), dim3(256), 0, n->stream, n->fields, n->x.a, (long long)P, 32);
@@ -1194,6 +1220,7 @@ int lbm_unet_loss_grad(lbm_unet *n, const float *x, const float *y, const float 
     return uguard([&] {
         ucheck(n && x && y && weights, "null argument");
         HIPCHK(hipSetDevice(n->dev));
+        n->order_after_last_use(n->stream);
         const float l = n->loss_and_grad(x, y, weights);
         if (loss) *loss = l;
         if (grad) {
@@ -1207,6 +1234,7 @@ int lbm_unet_train_step(lbm_unet *n, const float *x, const float *y, const float
     return uguard([&] {
         ucheck(n && x && y && weights, "null argument");
         HIPCHK(hipSetDevice(n->dev));
+        n->order_after_last_use(n->stream);
         const float l = n->loss_and_grad(x, y, weights);
         n->adam(lr);
         HIPCHK(hipStreamSynchronize(n->stream));
@@ -1218,6 +1246,7 @@ int lbm_unet_time_forward(lbm_unet *n, int reps, float *ms) {
     return uguard([&] {
         ucheck(n && ms && reps > 0, "bad argument");
         HIPCHK(hipSetDevice(n->dev));
+        n->order_after_last_use(n->stream);
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));

@@ -1,0 +1,255 @@
+"""The benchmark configurations at their own sizes on the GPU.
+
+  * plain FP32 channel flow at 8192 x 4096 (200 seeded circles r 8..64, seed
+    2024, tau 0.53, u_in 0.08): every step-kernel shape the autotuner can pick,
+    the two-step kernel included, against the one-step kernel on both state sets
+    and the renderer's moments, and the headline kernel against the oracle;
+  * the same flow at 4096 x 2048 against the oracle bit for bit, and the step at
+    which the flow leaves BGK's stable range, shown to be the oracle's own;
+  * FP16-shifted storage at 8192 x 4096 and 16384 x 8192 against the FP64
+    solver (itself bit-exact with the CPU FP64 oracle);
+  * 512 batched domains against single grids and the oracle;
+  * the surrogate rollout on 1024 x 512 against plain LBM.
+
+The step these stand in for is LBM::compute_step (lbm-wgpu src/lbm.rs:1112-1134)
+at those sizes; its f32 arithmetic is the oracle's (oracle/refcpu.c).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from test_gpu_parity import bits, both_sets, same_bits
+
+pytestmark = pytest.mark.gpu
+
+SEED = 2024
+TAU2, U2 = 0.53, 0.08
+
+
+def c2_mask(X, Y):
+    from lbm_amd import gen_circles
+    return gen_circles(X, Y, SEED, 200, 8, 64)
+
+
+def _grid(X, Y, mask, precision=0, variant=0, tau=TAU2, u=U2):
+    from lbm_amd import LBM
+    g = LBM(1.0 / tau, X, Y, precision=precision)
+    g.mask = mask
+    g.custom_speed(u)
+    g.variant = variant
+    return g
+
+
+def _compare_sets(a, b, what):
+    sa = both_sets(a)
+    sb = both_sets(b)
+    if not np.array_equal(sa, sb):
+        d = np.argwhere(sa != sb)
+        pytest.fail(f"{what}: {len(d)} differing values, first at (set,y,k,x)={tuple(d[0])}")
+    del sa, sb
+    for ma, mb in zip(a.moments(True), b.moments(True)):
+        assert same_bits(ma, mb), f"{what}: renderer moments differ"
+
+
+@pytest.mark.parametrize("precision,variants", [(0, (8, 9, 1, 3)), (1, (4, 7, 8))])
+def test_config2_8192x4096_kernel_shapes_equal_one_step(precision, variants):
+    """8192 x 4096: 33 strips x 512 row blocks with the XCD-aware block order
+    for the two-step kernel (about 16.9 k blocks), the several-cells-per-thread
+    one-step shapes, after a call of odd and of even length: both parity sets,
+    the renderer's moments and the FP64 sums equal the one-step kernel's bit for
+    bit.  FP32: the two-step kernel's state after 19 steps equals the oracle's."""
+    print(f"case: config 2 8192x4096 precision {precision} variants {variants}", flush=True)
+    X, Y = 8192, 4096
+    mask = c2_mask(X, Y)
+    a = _grid(X, Y, mask, precision, 0)
+    a.step(9)
+    a.step(10)
+    for v in variants:
+        print(f"  variant {v}", flush=True)
+        b = _grid(X, Y, mask, precision, v)
+        b.step(9)
+        b.step(10)
+        _compare_sets(a, b, f"variant {v} after 19 steps")
+        assert np.array_equal(a.conservation(), b.conservation())
+        if precision == 0 and v == 8:
+            r = oracle.RefCPU(X, Y, 1.0 / TAU2)
+            r.set_mask(mask.astype(np.uint32))
+            r.reset_equilibrium(1.0, U2, 0.0)
+            r.step(19)
+            assert same_bits(b.read_f(), r.state()), "two-step kernel vs oracle at 8192x4096"
+            jx, jy, rho = b.moments(True)
+            rjx, rjy, rrho = r.density()
+            assert same_bits(jx, rjx) and same_bits(jy, rjy) and same_bits(rho, rrho)
+            del r
+        b.close()
+    a.close()
+
+
+def _fluid_finite(f9, mask):
+    return bool(np.isfinite(f9[:, mask != 1]).all())
+
+
+def test_config2_4096x2048_bit_exact_and_divergence_is_the_references():
+    """4096 x 2048, autotuned-at-this-size one-step shape (variant 1): the whole
+    state equals the oracle bit for bit after 200 steps.  At tau 0.53 with 200
+    circles on this smaller grid the flow leaves BGK's stable range: the GPU
+    finds the first step whose state is non-finite on a fluid cell; the oracle,
+    run to the step before, holds the same state bit for bit and is finite, and
+    one step later is non-finite on exactly the same cells.  So the divergence
+    is the reference arithmetic's, not the kernel's."""
+    print("case: config 2 4096x2048", flush=True)
+    from lbm_amd import LbmError
+    X, Y = 4096, 2048
+    mask = c2_mask(X, Y)
+    g = _grid(X, Y, mask, 0, 1)
+    r = oracle.RefCPU(X, Y, 1.0 / TAU2)
+    r.set_mask(mask.astype(np.uint32))
+    r.reset_equilibrium(1.0, U2, 0.0)
+    g.step(200)
+    r.step(200)
+    assert same_bits(g.read_f(), r.state()), "4096x2048 after 200 steps"
+    # the first non-finite step on the GPU (FP64 sums over fluid cells)
+    limit, first = 6000, None
+    for n in range(201, limit + 1):
+        g.step(1)
+        try:
+            g.conservation()
+        except LbmError:
+            first = n
+            break
+    print(f"  first non-finite step on the GPU: {first}", flush=True)
+    if first is None:
+        r.step(limit - 200)
+        assert same_bits(g.read_f(), r.state())
+        pytest.skip(f"the flow stayed finite for {limit} steps: state equal to the oracle's, no divergence")
+    h = _grid(X, Y, mask, 0, 1)
+    h.step(first - 1)
+    r.step(first - 1 - 200)
+    fg = h.read_f()
+    fr = r.state()
+    assert _fluid_finite(fr, mask)
+    assert same_bits(fg, fr), f"state before the divergence (step {first - 1})"
+    del fg, fr
+    r.step(1)
+    fr = r.state()
+    fg = g.read_f()
+    assert not _fluid_finite(fr, mask), "the oracle stays finite where the GPU does not"
+    assert np.array_equal(np.isfinite(fg), np.isfinite(fr)), "non-finite cells differ"
+    fin = np.isfinite(fr)
+    assert np.array_equal(bits(fg)[fin], bits(fr)[fin])
+    g.close()
+    h.close()
+
+
+def _u_err(g, ux64, uy64, fl):
+    _, ux, uy, _ = g.fields()
+    num = ((ux - ux64) ** 2 + (uy - uy64) ** 2)[fl].sum()
+    return float(np.sqrt(num / (ux64 ** 2 + uy64 ** 2)[fl].sum()))
+
+
+@pytest.mark.parametrize("X,Y,tau,u,budget", [(8192, 4096, 0.6, 0.05, 1e-3), (16384, 8192, 0.6, 0.05, 1e-3)])
+def test_config3_f16s_against_fp64_full_size(X, Y, tau, u, budget):
+    """FP16-shifted storage at the configuration's sizes (200 circles r 8..64,
+    seed 2024), 1000 steps, against the FP64 solver: velocity rel-L2 over fluid
+    cells within the budget, the f32 grid (the floor) at least 10x closer, and
+    the FP16 grid's FP64 mass sum within 1e-4 of the FP64 solver's."""
+    print(f"case: config 3 f16s {X}x{Y} tau {tau} u {u}", flush=True)
+    from lbm_amd import LBM64
+    mask = c2_mask(X, Y)
+    fl = mask != 1
+    ref = LBM64(1.0 / tau, X, Y, u)
+    ref.set_mask(mask)
+    ref.step(1000)
+    rho64, ux64, uy64 = ref.macro()
+    m64 = float(rho64[fl].sum())
+    ref.close()
+    del rho64
+    errs = {}
+    for p in (0, 1):
+        g = _grid(X, Y, mask, p, 4 if p else 0, tau, u)
+        g.step(1000)
+        errs[p] = _u_err(g, ux64, uy64, fl)
+        if p == 1:
+            mass = float(g.conservation()[0])
+            assert abs(mass - m64) / m64 < 1e-4
+        g.close()
+    print(f"  u rel-L2 vs FP64 after 1000 steps: f32 {errs[0]:.3e}, f16s {errs[1]:.3e}", flush=True)
+    assert errs[1] < budget
+    assert errs[0] < errs[1] / 10
+
+
+def test_config4_512_batched_domains():
+    """512 domains (seed 2024: 128x64 .. 1024x512, tau 0.52..0.8, u_in
+    0.02..0.1, 1-12 circles/rectangles each), autotuned tile shape, hipGraph
+    replay, 20 steps: eight domains spread over the batch equal single grids of
+    their size bit for bit, and one of them equals the oracle."""
+    print("case: config 4 512 domains", flush=True)
+    from lbm_amd import LBM, Batch, gen_batch, gen_shapes
+    doms = gen_batch(SEED, 512)
+    masks = [gen_shapes(nx, ny, 10_000 + i, 1, 12) for i, (nx, ny, _, _) in enumerate(doms)]
+    b = Batch(doms)
+    for i, m in enumerate(masks):
+        b.set_mask(i, m)
+    b.step(3)
+    best, _ = b.autotune(4)
+    b.set_graph(True)
+    b.step(17)
+    assert b.step_count == 20
+    picks = [0, 63, 128, 200, 255, 311, 400, 511]
+    for i in picks:
+        nx, ny, om, u = doms[i]
+        g = LBM(om, nx, ny, u_init=u)
+        g.mask = masks[i]
+        g.step(20)
+        assert same_bits(b.read_f(i), g.read_f()), f"domain {i} ({nx}x{ny}), tile variant {best}"
+        for ma, mg in zip(b.moments(i), g.moments(lagged=True)):
+            assert same_bits(ma, mg), f"moments of domain {i}"
+        g.close()
+    nx, ny, om, u = doms[311]
+    r = oracle.RefCPU(nx, ny, om)
+    r.set_mask(masks[311].astype(np.uint32))
+    r.reset_equilibrium(1.0, u, 0.0)
+    r.step(20)
+    assert same_bits(b.read_f(311), r.state())
+    b.close()
+
+
+def test_config5_rollout_three_cycles():
+    """1024 x 512 (12 circles r 8..32, seed 2024, tau 0.6, u_in 0.05, after
+    2000 plain steps): three surrogate cycles (jump 64 with the trained
+    half-resolution network, 8 correctors) against 216 plain steps from the same
+    state: velocity rel-L2 under 1%, and under half the error of the same
+    cycles with the network's change set to zero."""
+    print("case: config 5 rollout", flush=True)
+    import os
+    from lbm_amd import LBM, gen_circles
+    from lbm_amd.rollout import Rollout, velocity_error
+    from lbm_amd.surrogate import UNet, load_params
+    X, Y, tau, u = 1024, 512, 0.6, 0.05
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lbm_amd", "data",
+                        "surrogate_f2.npy")
+    mask = gen_circles(X, Y, SEED, 12, 8, 32)
+
+    def make():
+        g = LBM(1.0 / tau, X, Y)
+        g.mask = mask
+        g.custom_speed(u)
+        g.step(2000)
+        return g
+    net = UNet(1, Y // 2, X // 2, seed=1)
+    params = load_params(path)
+    errs = {}
+    for label, p in (("trained", params), ("zero", np.zeros_like(params))):
+        net.set_params(p)
+        a, b = make(), make()
+        st = Rollout(a, net, threshold=float("inf")).run(3 * 72)
+        assert st.jumps == 3 and st.fallbacks == 0
+        b.step(3 * 72)
+        errs[label] = velocity_error(a.fields(), b.fields(), mask)
+        assert a.get_compute_num() == b.get_compute_num()
+        a.close()
+        b.close()
+    net.close()
+    print(f"  u rel-L2 after 3 cycles: trained {errs['trained']:.4f}, zero change {errs['zero']:.4f}", flush=True)
+    assert errs["trained"] < 0.01
+    assert errs["trained"] < 0.5 * errs["zero"]

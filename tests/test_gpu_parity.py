@@ -400,6 +400,11 @@ def test_two_step_variants_whole_state_and_paint_between_calls(variant, precisio
         grids.append(g)
     a, b = grids
     calls = [1, 7, 10, 2, 1, 5, 4]
+    # paints after calls that launched the two-step kernel (7 and 10 steps) and
+    # after one that did not (2 steps): newly solid cells read the other set
+    paints = {1: [1000 + 15 * X, 1, 1001 + 15 * X, 1, 1800 + 20 * X, 1],
+              2: [600 + 12 * X, 1, 1000 + 15 * X, 0, 2000 + 8 * X, 1],
+              3: [1400 + 25 * X, 1, 601 + 12 * X, 1]}
     for i, n in enumerate(calls):
         a.step(n)
         b.step(n)
@@ -408,8 +413,8 @@ def test_two_step_variants_whole_state_and_paint_between_calls(variant, precisio
             d = np.argwhere(sa != sb)
             pytest.fail(f"call {i} ({n} steps): {len(d)} differing values, first at (set,y,k,x)={tuple(d[0])}")
         assert same_bits(a.moments(True)[2], b.moments(True)[2])
-        if i == 3:
-            pairs = np.array([1000 + 15 * X, 1, 1001 + 15 * X, 1, 1800 + 20 * X, 1], np.uint32)
+        if i in paints:
+            pairs = np.array(paints[i], np.uint32)
             a.paint(pairs)
             b.paint(pairs)
     assert same_bits(a.read_f(), b.read_f())
@@ -417,9 +422,10 @@ def test_two_step_variants_whole_state_and_paint_between_calls(variant, precisio
         r = oracle.RefCPU(X, Y, 1.75)
         r.set_mask(mask.astype(np.uint32))
         r.reset_equilibrium(1.0, 0.09, 0.0)
-        r.step(sum(calls[:4]))
-        assert r.paint(pairs) == 0
-        r.step(sum(calls[4:]))
+        for i, n in enumerate(calls):
+            r.step(n)
+            if i in paints:
+                assert r.paint(np.array(paints[i], np.uint32)) == 0
         assert_state_equal(b, r, f"two-step variant {variant}")
     a.close()
     b.close()
