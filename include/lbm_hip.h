@@ -44,6 +44,12 @@ extern "C" {
 /* storage of the distributions; arithmetic is always f32 */
 #define LBM_PRECISION_F32 0  /* 9 x f32 per cell, 72 B per cell update */
 #define LBM_PRECISION_F16S 1 /* f_k - w_k as binary16 (round to nearest even), 36 B */
+/* f_k - feq_k(1, (u_in, 0)) as binary16, 36 B: shifted by the free-stream
+ * equilibrium of the inflow speed u_in of the last uniform reset (lbm_create's
+ * u_init, lbm_reset_equilibrium's ux), so a flow near the free stream stores
+ * small values and loses less to binary16 rounding.  The same kernels as F16S;
+ * a reset to another speed re-centres the storage (both sets are refilled). */
+#define LBM_PRECISION_F16E 2
 
 /* outlet treatment */
 #define LBM_BOUNDARY_REFERENCE 0     /* the reference's flattened-index wrap at x = nx-1 */
@@ -189,6 +195,7 @@ int lbm_render(lbm_grid *g, int kind, int colormap, float *rgb);
 /* FP64 sums of rho, jx, jy over the fluid cells (mask != 1) of the current
  * state.  Returns LBM_ENONFINITE if any sum is NaN or Inf. */
 int lbm_conservation(lbm_grid *g, double out[3]);
+
 
 /* ---- state ------------------------------------------------------------------------- */
 

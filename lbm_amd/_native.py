@@ -23,6 +23,7 @@ BATCH_NUM_VARIANTS = 6
 
 PRECISION_F32 = 0
 PRECISION_F16S = 1
+PRECISION_F16E = 2  # binary16 of f - feq(1, (u_in, 0)): centred on the free stream
 BOUNDARY_REFERENCE = 0
 BOUNDARY_ZERO_GRADIENT = 1
 

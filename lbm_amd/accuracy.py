@@ -5,7 +5,8 @@
 Workload: the benchmark's (200 seeded circles r 8..64, seed 2024, tau 0.53,
 u_in 0.08, reference outlet) at the given size; all three solvers start from
 feq(1, (u_in, 0)).  At each checkpoint one JSON line gives, per precision, the
-relative L2 over fluid cells of the velocity and of rho - 1 against FP64, and
+relative L2 over fluid cells of the velocity and of rho - 1 against FP64 (f32,
+FP16 of f - w, FP16 of f - feq(1, u_in)), and
 the FP64 sums of mass and x/y momentum over fluid cells (the grid's own device
 reduction) with their relative deviation from the FP64 solver's sums.  The
 open boundaries exchange mass, so "drift" is measured against the FP64
@@ -20,7 +21,7 @@ import time
 
 import numpy as np
 
-from . import LBM, LBM64, PRECISION_F16S, PRECISION_F32, LbmError, gen_circles
+from . import LBM, LBM64, PRECISION_F16E, PRECISION_F16S, PRECISION_F32, LbmError, gen_circles
 
 
 def rel_l2(a, b, fl):
@@ -33,7 +34,7 @@ def run(nx, ny, checkpoints, tau=0.53, u_in=0.08, seed=2024, circles=200, r_min=
     ref = LBM64(1.0 / tau, nx, ny, u_in)
     ref.set_mask(mask)
     grids = {}
-    for name, p in (("f32", PRECISION_F32), ("f16s", PRECISION_F16S)):
+    for name, p in (("f32", PRECISION_F32), ("f16s", PRECISION_F16S), ("f16e", PRECISION_F16E)):
         g = LBM(1.0 / tau, nx, ny, precision=p)
         g.mask = mask
         g.custom_speed(u_in)

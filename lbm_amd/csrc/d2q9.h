@@ -16,6 +16,8 @@
 
 #include <stdint.h>
 
+#include <cmath>
+
 namespace d2q9 {
 
 enum : int { NW = 0, N = 1, NE = 2, W = 3, C = 4, E = 5, SW = 6, S = 7, SE = 8 };
@@ -149,14 +151,46 @@ __device__ __forceinline__ Moments collide_shifted(float g[9], float omega) {
 // F16S g_k = f_k - w_k as IEEE binary16, converted with round-to-nearest-even,
 // collided in f32 in shifted form.  to_f / from_f convert to and from f_k for
 // uploads, downloads and fills.
+//
+// The binary16 storage may also hold f_k - s_k with s_k = feq_k(1, (u_s, 0)),
+// the free-stream equilibrium of a channel whose inflow speed is u_s: where the
+// flow is close to the free stream the stored values are then small, and
+// binary16's relative rounding error costs proportionally less velocity.  The
+// arithmetic stays in the variables of f - w: a value is shifted by
+// d_k = s_k - w_k when it is loaded and back before it is rounded.  on = 0
+// (and d = 0) is the plain f - w format.
+//
+// d_k is rounded to a multiple of 2^-20.  Every binary16 value is a multiple of
+// 2^-24, so for stored values below 16 in magnitude the f32 sum h + d_k (and
+// h + d_opp - d_k) is exact: decoding loses nothing, re-encoding an unchanged
+// value gives back the same bits, and a value that half-way bounce-back hands
+// from population opp(k) to k is re-encoded the same way on every code path.
+struct F16Shift {
+    float d[9];
+    int on;
+};
+
+inline F16Shift make_f16_shift(bool on, double us) {
+    F16Shift s{};
+    s.on = on ? 1 : 0;
+    if (!on) return s;
+    for (int k = 0; k < 9; ++k) {
+        const double w = k == 4 ? 4.0 / 9.0 : ((k & 1) ? 1.0 / 9.0 : 1.0 / 36.0);
+        const double cu = (double)((k % 3) - 1) * us;  // the shift flow is along +x
+        const double d = w * (3.0 * cu + 4.5 * cu * cu - 1.5 * us * us);
+        s.d[k] = (float)(std::nearbyint(d * 1048576.0) / 1048576.0);
+    }
+    return s;
+}
 
 struct StoreF32 {
     using T = float;
     static constexpr int bytes = 4;
-    __device__ __forceinline__ static float load(T v) { return v; }
-    __device__ __forceinline__ static T store(float w) { return w; }
-    __device__ __forceinline__ static float to_f(T v, int) { return v; }
-    __device__ __forceinline__ static T from_f(float f, int) { return f; }
+    __device__ __forceinline__ static float load(T v, int, const F16Shift &) { return v; }
+    __device__ __forceinline__ static float load_b(T v, int, bool, const F16Shift &) { return v; }
+    __device__ __forceinline__ static T store(float w, int, const F16Shift &) { return w; }
+    __device__ __forceinline__ static float to_f(T v, int, const F16Shift &) { return v; }
+    __device__ __forceinline__ static T from_f(float f, int, const F16Shift &) { return f; }
     __device__ __forceinline__ static Moments collide(float w[9], float omega) { return collide_ref(w, omega); }
     __device__ __forceinline__ static Moments moments(const float w[9], bool with_rest) {
         return moments_ref(w, with_rest);
@@ -168,21 +202,33 @@ struct StoreF16S {
     static constexpr int bytes = 2;
     // load is pinned like store: every kernel shape then performs exactly one
     // f16 -> f32 conversion per value, never a mixed-precision FMA in its place.
-    __device__ __forceinline__ static float load(T v) {
+    __device__ __forceinline__ static float load(T v, int k, const F16Shift &sh) {
         float g = (float)__builtin_bit_cast(_Float16, v);
         asm("" : "+v"(g));
+        if (sh.on) g += sh.d[k];
+        return g;
+    }
+    // population k's working value from a stored value of population opp(k) when
+    // `bounced` (half-way bounce-back hands a cell its own opposite population)
+    __device__ __forceinline__ static float load_b(T v, int k, bool bounced, const F16Shift &sh) {
+        float g = (float)__builtin_bit_cast(_Float16, v);
+        asm("" : "+v"(g));
+        if (sh.on) g += bounced ? sh.d[opp(k)] : sh.d[k];
         return g;
     }
     // The empty asm pins g as an f32 register value, so the conversion is always
     // a separate f32 -> f16 round to nearest even; otherwise the compiler may fuse
     // it with the producing FMA (one rounding instead of two) in some kernel
     // shapes and not others, and kernels would disagree in the last bit.
-    __device__ __forceinline__ static T store(float g) {
+    __device__ __forceinline__ static T store(float g, int k, const F16Shift &sh) {
+        if (sh.on) g -= sh.d[k];
         asm("" : "+v"(g));
         return __builtin_bit_cast(uint16_t, (_Float16)g);
     }
-    __device__ __forceinline__ static float to_f(T v, int k) { return load(v) + weight(k); }
-    __device__ __forceinline__ static T from_f(float f, int k) { return store(f - weight(k)); }
+    __device__ __forceinline__ static float to_f(T v, int k, const F16Shift &sh) { return load(v, k, sh) + weight(k); }
+    __device__ __forceinline__ static T from_f(float f, int k, const F16Shift &sh) {
+        return store(f - weight(k), k, sh);
+    }
     __device__ __forceinline__ static Moments collide(float w[9], float omega) { return collide_shifted(w, omega); }
     __device__ __forceinline__ static Moments moments(const float w[9], bool with_rest) {
         return moments_shifted(w, with_rest);

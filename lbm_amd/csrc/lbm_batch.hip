@@ -393,7 +393,7 @@ int lbm_batch_create(const lbm_domain *doms, uint32_t n, int32_t precision, int3
                      lbm_batch **out) {
     if (!doms || !out || n == 0) return bfail(nullptr, LBM_EINVAL, "need at least one domain");
     *out = nullptr;
-    if (precision != LBM_PRECISION_F32 && precision != LBM_PRECISION_F16S)
+    if (precision != LBM_PRECISION_F32 && precision != LBM_PRECISION_F16S && precision != LBM_PRECISION_F16E)
         return bfail(nullptr, LBM_EINVAL, "unknown precision");
     if (boundary != LBM_BOUNDARY_REFERENCE && boundary != LBM_BOUNDARY_ZERO_GRADIENT)
         return bfail(nullptr, LBM_EINVAL, "unknown boundary mode");
@@ -427,6 +427,7 @@ int lbm_batch_create(const lbm_domain *doms, uint32_t n, int32_t precision, int3
         d.aux_off = ao;
         d.seg_off = so;
         d.g = Geom{X, Y, pitch, pst, 9LL * pitch, (long long)pitch, doms[i].omega};
+        d.g.sh = make_f16_shift(precision == LBM_PRECISION_F16E, doms[i].u_in);  // each domain's own free stream
         b->table.push_back(d);
         fo += (9LL * pitch * Y + 255) / 256 * 256;
         ao += pst;

@@ -28,6 +28,7 @@ struct Geom {
     long long pstride;  // elements per auxiliary plane (mask, flags, moments)
     long long ys, ks;   // state strides: element (k, x, y) of a set is at y * ys + k * ks + x
     float omega;
+    F16Shift sh;        // shift of the binary16 storage (off: f - w)
 };
 
 // per-cell bounce-back byte: bit(k) set if population k's upstream is solid or
@@ -178,7 +179,7 @@ __device__ __forceinline__ bool gather_cell(const typename St::T *src, const typ
                 upstream<MODE>(k, x, y, g.X, ux, uy);
                 const bool bounce = (fl >> bit(k)) & 1u;
                 const size_t idx = bounce ? (size_t)opp(k) * ks + sown : (size_t)k * ks + (size_t)uy * g.ys + ux;
-                f[k] = St::load(src[idx]);
+                f[k] = St::load_b(src[idx], k, bounce, g.sh);
             }
         } else {
             // enclosed streamed cell: every population bounces back; otherwise
@@ -187,13 +188,14 @@ __device__ __forceinline__ bool gather_cell(const typename St::T *src, const typ
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 if (k == C) continue;
-                f[k] = streamed ? St::load(src[(size_t)opp(k) * ks + sown]) : St::load(dst[(size_t)k * ks + sown]);
+                f[k] = St::load_b(streamed ? src[(size_t)opp(k) * ks + sown] : dst[(size_t)k * ks + sown], k,
+                                  streamed, g.sh);
             }
         }
-        f[C] = St::load(src[(size_t)C * ks + sown]);
+        f[C] = St::load(src[(size_t)C * ks + sown], C, g.sh);
     } else {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) f[k] = St::load(src[(size_t)k * ks + sown]);
+        for (int k = 0; k < 9; ++k) f[k] = St::load(src[(size_t)k * ks + sown], k, g.sh);
     }
     return streamed;
 }
@@ -211,14 +213,14 @@ __device__ __forceinline__ void finish_cell(float f[9], bool streamed, const typ
         if (streamed) {
 #pragma unroll
             for (int k = 0; k < 9; ++k)
-                if (k != C) dst[(size_t)k * ks + sown] = St::store(f[k]);
+                if (k != C) dst[(size_t)k * ks + sown] = St::store(f[k], k, g.sh);
         }
         dst[(size_t)C * ks + sown] = src[(size_t)C * ks + sown];
     }
     if (OP & OP_COLLIDE) {
         m = St::collide(f, g.omega);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) dst[(size_t)k * ks + sown] = St::store(f[k]);
+        for (int k = 0; k < 9; ++k) dst[(size_t)k * ks + sown] = St::store(f[k], k, g.sh);
     } else {
         m = St::moments(f, !(OP & OP_NOREST));
     }
@@ -301,12 +303,12 @@ __device__ __forceinline__ void interior_tile(const typename St::T *src, typenam
     for (int j = 0; j < CPT; ++j) {
         float f[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) f[k] = St::load(get_lane<CPT>(v[k], j));
+        for (int k = 0; k < 9; ++k) f[k] = St::load(get_lane<CPT>(v[k], j), k, g.sh);
         Moments m;
         if (OP & OP_COLLIDE) {
             m = St::collide(f, g.omega);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) set_lane<CPT>(o[k], j, St::store(f[k]));
+            for (int k = 0; k < 9; ++k) set_lane<CPT>(o[k], j, St::store(f[k], k, g.sh));
         } else {
             m = St::moments(f, !(OP & OP_NOREST));
         }
@@ -413,6 +415,13 @@ __device__ __forceinline__ void patched_tile(const typename St::T *src, typename
                 if (j == jo) set_lane<CPT>(v[k], j, src[(size_t)k * ks + (size_t)uy * g.ys + ux]);
         }
     }
+    // from_opp[j] bit k: lane j of v[k] holds the cell's own population opp(k)
+    // (bounce-back), which the shifted binary16 format decodes with opp(k)'s shift
+    uint32_t from_opp[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) from_opp[j] = 0;
+    [[maybe_unused]] float fe[9];
+    [[maybe_unused]] bool gathered = false;
     if (fw != 0 || (kGatherEdges && edge)) {
 #pragma unroll
         for (int j = 0; j < CPT; ++j) {
@@ -421,10 +430,15 @@ __device__ __forceinline__ void patched_tile(const typename St::T *src, typename
             const uint32_t fl = (uint32_t)(fw >> (8 * j)) & 0xFFu;
             const size_t sown = sown0 + j;
             if (kGatherEdges && edge) {
-                float f[9];
-                gather_cell<St, MODE, true>(src, dst, flags, mask, g, x, y, own0 + j, sown, f);
+                if constexpr (St::bytes == 2) {
+                    gather_cell<St, MODE, true>(src, dst, flags, mask, g, x, y, own0 + j, sown, fe);
+                    gathered = true;
+                } else {
+                    float f[9];
+                    gather_cell<St, MODE, true>(src, dst, flags, mask, g, x, y, own0 + j, sown, f);
 #pragma unroll
-                for (int k = 0; k < 9; ++k) set_lane<CPT>(v[k], j, St::store(f[k]));
+                    for (int k = 0; k < 9; ++k) set_lane<CPT>(v[k], j, St::store(f[k], k, g.sh));
+                }
             } else if (fl == kSlowCell) {
                 const bool streamed = mask[own0 + j] != 1 && x != 0 && y < g.Y - 1;
 #pragma unroll
@@ -432,11 +446,15 @@ __device__ __forceinline__ void patched_tile(const typename St::T *src, typename
                     if (k == C) continue;
                     set_lane<CPT>(v[k], j, streamed ? src[(size_t)opp(k) * ks + sown] : dst[(size_t)k * ks + sown]);
                 }
+                if (streamed) from_opp[j] = 0x1FFu & ~(1u << C);
             } else {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {
                     if (k == C) continue;
-                    if ((fl >> bit(k)) & 1u) set_lane<CPT>(v[k], j, src[(size_t)opp(k) * ks + sown]);
+                    if ((fl >> bit(k)) & 1u) {
+                        set_lane<CPT>(v[k], j, src[(size_t)opp(k) * ks + sown]);
+                        from_opp[j] |= 1u << k;
+                    }
                 }
             }
         }
@@ -447,12 +465,27 @@ __device__ __forceinline__ void patched_tile(const typename St::T *src, typename
     for (int j = 0; j < CPT; ++j) {
         float f[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) f[k] = St::load(get_lane<CPT>(v[k], j));
+        for (int k = 0; k < 9; ++k) {
+            if constexpr (kGatherEdges && St::bytes == 2) {
+                if (gathered) {  // no store/load round trip: the shifted format does not invert exactly
+                    f[k] = fe[k];
+                    continue;
+                }
+            }
+            f[k] = St::load_b(get_lane<CPT>(v[k], j), k, (from_opp[j] >> k) & 1u, g.sh);
+        }
+        if constexpr ((OP & OP_STORE_S) && St::bytes == 2) {
+            // the pulled state as stored values of each population: bounced
+            // values are re-encoded with their new population's shift (values
+            // that were not moved between populations come back bit for bit)
+#pragma unroll
+            for (int k = 0; k < 9; ++k) set_lane<CPT>(v[k], j, St::store(f[k], k, g.sh));
+        }
         Moments m;
         if (OP & OP_COLLIDE) {
             m = St::collide(f, g.omega);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) set_lane<CPT>(o[k], j, St::store(f[k]));
+            for (int k = 0; k < 9; ++k) set_lane<CPT>(o[k], j, St::store(f[k], k, g.sh));
         } else {
             m = St::moments(f, !(OP & OP_NOREST));
         }
@@ -488,7 +521,7 @@ __global__ void k_fill(typename St::T *buf, Geom g, float4 v0123, float4 v4567, 
     if (i >= (size_t)g.ys * g.Y) return;
     const float v[9] = {v0123.x, v0123.y, v0123.z, v0123.w, v4567.x, v4567.y, v4567.z, v4567.w, v8};
     const int k = (int)((i / (size_t)g.pitch) % 9);
-    buf[i] = St::from_f(v[k], k);
+    buf[i] = St::from_f(v[k], k, g.sh);
 }
 
 // population k of a set (plane points at element (k, 0, 0)) <-> a dense f32 plane
@@ -497,7 +530,7 @@ __global__ void k_encode_plane(const float *__restrict__ dense, typename St::T *
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= g.X) return;
-    plane[(size_t)y * g.ys + x] = St::from_f(dense[(size_t)y * g.X + x], k);
+    plane[(size_t)y * g.ys + x] = St::from_f(dense[(size_t)y * g.X + x], k, g.sh);
 }
 
 template <class St>
@@ -505,12 +538,12 @@ __global__ void k_decode_plane(const typename St::T *__restrict__ plane, float *
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= g.X) return;
-    dense[(size_t)y * g.X + x] = St::to_f(plane[(size_t)y * g.ys + x], k);
+    dense[(size_t)y * g.X + x] = St::to_f(plane[(size_t)y * g.ys + x], k, g.sh);
 }
 
 template <class St>
-__global__ void k_set_one(typename St::T *p, float v, int k) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = St::from_f(v, k);
+__global__ void k_set_one(typename St::T *p, float v, int k, F16Shift sh) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = St::from_f(v, k, sh);
 }
 
 __global__ void k_paint(const uint32_t *__restrict__ idx, const uint8_t *__restrict__ val, size_t n,

@@ -100,7 +100,7 @@ __global__ void k_pair_col0(const typename St::T *A, const typename St::T *B, co
     gather_cell<St, MODE, true>(A, B, flags, mask, g, 0, r, (size_t)r * g.pitch, (size_t)r * g.ys, f);
     St::collide(f, g.omega);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) col0[(size_t)k * g.Y + r] = St::store(f[k]);
+    for (int k = 0; k < 9; ++k) col0[(size_t)k * g.Y + r] = St::store(f[k], k, g.sh);
 }
 
 // Barrier for LDS hand-off only: waits for this wave's LDS operations, not for
@@ -164,19 +164,20 @@ __global__ __launch_bounds__(kBlock) void k_pair(const typename St::T *A, const 
             streamed = gather_cell<St, MODE, true>(A, B, flags, mask, g, x, r, own, sown, f);
         } else {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) f[k] = St::load(v[k]);
+            for (int k = 0; k < 9; ++k) f[k] = St::load(v[k], k, g.sh);
             if (fl == kSlowCell) {
                 streamed = mask[own] != 1 && r < g.Y - 1;
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {
                     if (k == C) continue;
-                    f[k] = St::load(streamed ? A[(size_t)opp(k) * ks + sown] : B[(size_t)k * ks + sown]);
+                    f[k] = St::load_b(streamed ? A[(size_t)opp(k) * ks + sown] : B[(size_t)k * ks + sown], k,
+                                      streamed, g.sh);
                 }
             } else if (fl != 0) {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {
                     if (k == C) continue;
-                    if ((fl >> bit(k)) & 1u) f[k] = St::load(A[(size_t)opp(k) * ks + sown]);
+                    if ((fl >> bit(k)) & 1u) f[k] = St::load(A[(size_t)opp(k) * ks + sown], opp(k), g.sh);
                 }
             }
         }
@@ -185,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(const typename St::T *A, const 
         T o[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            o[k] = St::store(f[k]);
+            o[k] = St::store(f[k], k, g.sh);
             if (k != C) ring[r & (kRing - 1)][bit(k)][t] = o[k];
         }
         if (store) {
@@ -205,37 +206,37 @@ __global__ __launch_bounds__(kBlock) void k_pair(const typename St::T *A, const 
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             if (k == C) continue;
-            f[k] = St::load(ring[(y - cy(k)) & (kRing - 1)][bit(k)][t - cx(k)]);
+            f[k] = St::load(ring[(y - cy(k)) & (kRing - 1)][bit(k)][t - cx(k)], k, g.sh);
         }
-        f[C] = St::load(rest);
+        f[C] = St::load(rest, C, g.sh);
         if (fl == kSlowCell) {
             const size_t sown = (size_t)y * g.ys + x;
             const bool streamed = mask[(size_t)y * g.pitch + x] != 1 && x != 0 && y < g.Y - 1;
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 if (k == C) continue;
-                f[k] = streamed ? St::load(ring[me][bit(opp(k))][t]) : St::load(A[(size_t)k * ks + sown]);
+                f[k] = St::load_b(streamed ? ring[me][bit(opp(k))][t] : A[(size_t)k * ks + sown], k, streamed, g.sh);
             }
         } else if (fl != 0 || x == g.X - 1) {
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 if (k == C) continue;
                 if ((fl >> bit(k)) & 1u) {
-                    f[k] = St::load(ring[me][bit(opp(k))][t]);
+                    f[k] = St::load(ring[me][bit(opp(k))][t], opp(k), g.sh);
                 } else if (x - cx(k) >= g.X) {
                     // the outlet's pull past the last column: the flattened
                     // index wraps to column 0 of the next row, or clamps
                     if constexpr (MODE == LBM_BOUNDARY_REFERENCE)
-                        f[k] = St::load(col0[(size_t)k * g.Y + (y - cy(k)) + 1]);
+                        f[k] = St::load(col0[(size_t)k * g.Y + (y - cy(k)) + 1], k, g.sh);
                     else
-                        f[k] = St::load(ring[(y - cy(k)) & (kRing - 1)][bit(k)][t]);
+                        f[k] = St::load(ring[(y - cy(k)) & (kRing - 1)][bit(k)][t], k, g.sh);
                 }
             }
         }
         St::collide(f, g.omega);
         T *dst = A2 + (size_t)y * g.ys;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(St::store(f[k]), dst + (size_t)k * ks + x);
+        for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(St::store(f[k], k, g.sh), dst + (size_t)k * ks + x);
     };
 
     // rows r0 - 1 and r0, then per row y: row y + 2's pulls issued, row y + 1
@@ -342,7 +343,7 @@ __global__ void k_apply_jump(const float *__restrict__ net_out, int C, const flo
     for (int k = 0; k < 9; ++k) {
         const float cu = (float)cx(k) * ux + (float)cy(k) * uym;
         const float feq = weight(k) * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
-        buf[(size_t)k * g.ks + sown] = St::from_f(feq, k);
+        buf[(size_t)k * g.ks + sown] = St::from_f(feq, k, g.sh);
     }
 }
 
@@ -398,6 +399,12 @@ struct lbm_grid {
     size_t esize = 4;
     float omega = 1.25f;
     float u_in = 0.1f;  // x velocity of the last uniform reset: the inlet column's speed
+
+    // the inflow speed also centres the F16E storage on the free-stream equilibrium
+    void set_inflow(float u) {
+        u_in = u;
+        geom.sh = make_f16_shift(precision == LBM_PRECISION_F16E, u);
+    }
     void *f[2] = {nullptr, nullptr};
     void *spare[2] = {nullptr, nullptr};  // output sets of the two-step kernel, allocated on first use
     void *col0 = nullptr;                 // its outlet-wrap column (9 x ny elements)
@@ -783,7 +790,8 @@ int lbm_create(const lbm_config *cfg, lbm_grid **out) {
     if (cfg->ny > 65535) return fail(nullptr, LBM_EINVAL, "ny must be <= 65535 (one launch row per grid row)");
     if ((uint64_t)cfg->nx * cfg->ny > 0xFFFFFFFFull)
         return fail(nullptr, LBM_EINVAL, "nx * ny must fit in 32 bits (the paint wire format is u32)");
-    if (cfg->precision != LBM_PRECISION_F32 && cfg->precision != LBM_PRECISION_F16S)
+    if (cfg->precision != LBM_PRECISION_F32 && cfg->precision != LBM_PRECISION_F16S &&
+        cfg->precision != LBM_PRECISION_F16E)
         return fail(nullptr, LBM_EINVAL, "unknown precision");
     if (cfg->boundary != LBM_BOUNDARY_REFERENCE && cfg->boundary != LBM_BOUNDARY_ZERO_GRADIENT)
         return fail(nullptr, LBM_EINVAL, "unknown boundary mode");
@@ -823,7 +831,7 @@ int lbm_create(const lbm_config *cfg, lbm_grid **out) {
         HIPCHK(hipMemsetAsync(g->out, 0, sizeof(float) * pstride, g->stream));
         float v[9];
         host_feq(cfg->u_init, 0.0f, 1.0f, v);
-        g->u_in = cfg->u_init;
+        g->set_inflow(cfg->u_init);
         g->fill_both(v);
         HIPCHK(hipMemsetAsync(g->mask, 0, pstride, g->stream));
         hipLaunchKernelGGL(k_walls, dim3((g->X + 255) / 256), dim3(256), 0, g->stream, g->mask, g->geom);
@@ -871,7 +879,7 @@ int lbm_reset_equilibrium(lbm_grid *g, float rho, float ux, float uy) {
     return guarded(g, [&] {
         float v[9];
         host_feq(ux, uy, rho, v);
-        g->u_in = ux;
+        g->set_inflow(ux);  // (F16E: re-centres the storage on the new free stream; both sets are refilled)
         g->fill_both(v);
         g->cur = 0;
         g->pform = false;
@@ -906,9 +914,11 @@ int lbm_single_cell(lbm_grid *g, int k) {
         for (int s = 0; s < 2; ++s) {
             char *p = (char *)g->f[s] + g->state_offset(k, x, y);
             if (g->precision == LBM_PRECISION_F32)
-                hipLaunchKernelGGL(k_set_one<StoreF32>, dim3(1), dim3(64), 0, g->stream, (float *)p, 4.0f, k);
+                hipLaunchKernelGGL(k_set_one<StoreF32>, dim3(1), dim3(64), 0, g->stream, (float *)p, 4.0f, k,
+                                   g->geom.sh);
             else
-                hipLaunchKernelGGL(k_set_one<StoreF16S>, dim3(1), dim3(64), 0, g->stream, (uint16_t *)p, 4.0f, k);
+                hipLaunchKernelGGL(k_set_one<StoreF16S>, dim3(1), dim3(64), 0, g->stream, (uint16_t *)p, 4.0f, k,
+                                   g->geom.sh);
             HIPCHK(hipGetLastError());
         }
     });
@@ -921,9 +931,10 @@ int lbm_set_pulse(lbm_grid *g, int k, uint32_t x, uint32_t y, float v) {
         g->flush();
         char *p = (char *)g->f[g->cur] + g->state_offset(k, x, y);
         if (g->precision == LBM_PRECISION_F32)
-            hipLaunchKernelGGL(k_set_one<StoreF32>, dim3(1), dim3(64), 0, g->stream, (float *)p, v, k);
+            hipLaunchKernelGGL(k_set_one<StoreF32>, dim3(1), dim3(64), 0, g->stream, (float *)p, v, k, g->geom.sh);
         else
-            hipLaunchKernelGGL(k_set_one<StoreF16S>, dim3(1), dim3(64), 0, g->stream, (uint16_t *)p, v, k);
+            hipLaunchKernelGGL(k_set_one<StoreF16S>, dim3(1), dim3(64), 0, g->stream, (uint16_t *)p, v, k,
+                               g->geom.sh);
         HIPCHK(hipGetLastError());
     });
 }
@@ -1430,7 +1441,7 @@ int lbm_load_state(lbm_grid *g, const void *buf, size_t bytes) {
         HIPCHK(hipMemcpyAsync(g->out, p, ps * sizeof(float), hipMemcpyHostToDevice, g->stream));
         g->omega = h.omega;
         g->geom.omega = h.omega;
-        g->u_in = h.u_in;
+        g->set_inflow(h.u_in);
         g->cur = h.cur;
         g->pform = h.pform != 0;
         g->steps = h.steps;

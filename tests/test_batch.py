@@ -90,7 +90,7 @@ def _small_batch():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 @pytest.mark.parametrize("variant", [0, 3, 5])
 def test_batch_domains_equal_single_grids(precision, variant):
     print(f"case: batch vs single grids precision {precision} variant {variant}", flush=True)

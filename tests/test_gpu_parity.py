@@ -287,6 +287,33 @@ def test_f16s_codec_round_trip():
     assert same_bits(got, exp)
 
 
+@pytest.mark.parametrize("u_in", [0.0, 0.08, -0.05])
+def test_f16e_codec_round_trip(u_in):
+    """F16E stores binary16 of f - feq(1, (u_in, 0)): per population the shift
+    d_k = feq_k(1, u_in) - w_k (computed in double, rounded to a multiple of
+    2^-20) is taken off
+    in f32 after w_k, then rounded to nearest even; decoding adds d_k, then
+    w_k.  At u_in = 0 it is the F16S format bit for bit."""
+    from lbm_amd import LBM, PRECISION_F16E
+    X, Y = 70, 30
+    rng = np.random.default_rng(2)
+    g = LBM(1.2, X, Y, precision=PRECISION_F16E, u_init=u_in)
+    w64 = np.array([1 / 36, 1 / 9, 1 / 36, 1 / 9, 4 / 9, 1 / 9, 1 / 36, 1 / 9, 1 / 36])
+    cx = np.array([-1, 0, 1, -1, 0, 1, -1, 0, 1], np.float64)
+    us = float(np.float32(u_in))  # the grid holds the inflow speed as f32
+    cu = cx * us
+    d = w64 * (3 * cu + 4.5 * cu * cu - 1.5 * us * us)
+    d = (np.rint(d * 2.0 ** 20) / 2.0 ** 20).astype(np.float32)[:, None, None]  # multiples of 2^-20
+    w = w64.astype(np.float32)[:, None, None]
+    f = (w * (1 + 0.2 * rng.standard_normal((9, Y, X)))).astype(np.float32)
+    g.write_f(f)
+    got = g.read_f()
+    exp = (((f - w) - d).astype(np.float16).astype(np.float32) + d) + w
+    assert same_bits(got, exp), f"{int((bits(got) != bits(exp)).sum())} values differ"
+    if u_in == 0.0:
+        assert same_bits(got, (f - w).astype(np.float16).astype(np.float32) + w)
+
+
 @pytest.mark.parametrize("steps", [1000])
 def test_f16s_error_vs_fp64_config1(steps):
     """FP16-shifted storage, FP32 collide: velocity error vs FP64 on config 1."""
@@ -304,7 +331,7 @@ def test_f16s_error_vs_fp64_config1(steps):
     fl = mask == 0
     e_u = np.sqrt((((gux - ux) ** 2 + (guy - uy) ** 2)[fl]).sum() / ((ux ** 2 + uy ** 2)[fl]).sum())
     print(f"config1 f16s vs fp64 after {steps}: rel-L2(u) = {e_u:.3e}")
-    assert e_u < 1e-2
+    assert e_u < 1e-3
 
 
 def test_full_size_properties_8192x4096():
@@ -337,7 +364,7 @@ def test_full_size_properties_8192x4096():
     assert np.isfinite(sa).all()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 @pytest.mark.parametrize("variant", list(range(10)))
 def test_every_kernel_variant_gives_the_same_bits(precision, variant):
     """The step-kernel variants move memory differently and must not change a bit.
@@ -379,7 +406,7 @@ def both_sets(g):
 
 
 @pytest.mark.parametrize("boundary", [0, 1])
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 @pytest.mark.parametrize("variant", [8, 9])
 def test_two_step_variants_whole_state_and_paint_between_calls(variant, precision, boundary):
     """Two steps per launch: after every call of odd and even length, both sets
@@ -441,7 +468,7 @@ def test_two_step_variants_small_and_boundary_sizes(X, Y):
     from lbm_amd import LBM
     circles = [(X // 2, Y // 2, max(1, min(X, Y) // 4))] if min(X, Y) >= 8 else []
     mask = circle_mask(X, Y, circles)
-    for precision in (0, 1):
+    for precision in (0, 1, 2):
         for boundary in (0, 1):
             for variant in (8, 9):
                 grids = []
@@ -497,7 +524,7 @@ def test_autotune_leaves_the_state_untouched():
     assert same_bits(a.moments(True)[0], b.moments(True)[0])
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 @pytest.mark.parametrize("cmap", [0, 1, 2])
 def test_render_colours_equal_colormap_restatement(cmap, precision):
     """lbm_render: the summary plane mapped to colours in the same pass equals
