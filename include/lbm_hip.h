@@ -196,6 +196,17 @@ int lbm_render(lbm_grid *g, int kind, int colormap, float *rgb);
  * state.  Returns LBM_ENONFINITE if any sum is NaN or Inf. */
 int lbm_conservation(lbm_grid *g, double out[3]);
 
+/* lbm_conservation without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate -- code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc -- where lbm_conservation
+ * would allocate its moment planes and partial-sum buffer on first use.  The
+ * partial sums go to a fixed device array of the library, 2^18 reduction
+ * blocks (6 MB) per device, which holds grids whose moments pass has at most
+ * that many blocks (f32: ceil(nx / 256) * ny; binary16: ceil(nx / 512) * ny
+ * once stepped); a larger grid is LBM_EINVAL.  Same pass and order as
+ * lbm_conservation: the same bits.  Every call on one device shares the array:
+ * calls that overlap on one device (grids on several threads) race. */
+int lbm_conservation_static(lbm_grid *g, double out[3]);
 
 /* ---- state ------------------------------------------------------------------------- */
 

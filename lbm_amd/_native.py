@@ -94,6 +94,7 @@ SIGNATURES = {
     "lbm_summary": (_i32, [_P, _i32, _fp]),
     "lbm_render": (_i32, [_P, _i32, _i32, _fp]),
     "lbm_conservation": (_i32, [_P, _dp]),
+    "lbm_conservation_static": (_i32, [_P, _dp]),
     "lbm_read_f": (_i32, [_P, _fp]),
     "lbm_write_f": (_i32, [_P, _fp]),
     "lbm_state_size": (_i32, [_P, ctypes.POINTER(_sz)]),

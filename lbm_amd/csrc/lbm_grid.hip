@@ -41,6 +41,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <unordered_map>
@@ -1329,6 +1330,65 @@ int lbm_conservation(lbm_grid *g, double out[3]) {
                            g->sums);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(out, g->sums, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        HIPCHK(hipStreamSynchronize(g->stream));
+    });
+    if (rc == LBM_OK && !(std::isfinite(out[0]) && std::isfinite(out[1]) && std::isfinite(out[2])))
+        return fail(g, LBM_ENONFINITE, "non-finite mass or momentum");
+    return rc;
+}
+
+}  // extern "C"
+
+// ---- FP64 sums without allocation ------------------------------------------------
+//
+// lbm_conservation_static's partial sums live in a file-scope device array of a
+// fixed size instead of buffers the grid allocates on first use.  In host code
+// `scratch` names a host-side stand-in, not device memory: its device address
+// comes from hipGetSymbolAddress, taken once per device (with that device
+// current) on the entry point's first call there, and only that address is
+// ever handed to a kernel.  Every call on one device shares the array, which
+// keeps the last call's contents (each call writes every slot it then reads).
+constexpr size_t kStaticSlots = (size_t)1 << 18;  // reduction blocks of one pass (6 MB of partial sums)
+static __device__ __align__(256) double scratch[3 * kStaticSlots + 4];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+double *scratch_on(int dev) {
+    check(dev >= 0 && dev < kMaxDevices, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));  // the caller made `dev` current
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int lbm_conservation_static(lbm_grid *g, double out[3]) {
+    int rc = guarded(g, [&] {
+        check(out != nullptr, "null output");
+        // the same pass shape as lbm_conservation's moments pass, so the same
+        // per-block partial sums in the same order: the same bits
+        const bool f16p = g->pform && g->precision != LBM_PRECISION_F32;
+        const size_t blocks = (size_t)g->shape(f16p ? 2 : 1).nblocks;
+        if (blocks > kStaticSlots)
+            throw ArgError{"grid too large for lbm_conservation_static: its reduction needs " +
+                           std::to_string(blocks) + " partial sums, the static scratch holds " +
+                           std::to_string(kStaticSlots) + " (use lbm_conservation)"};
+        double *part = scratch_on(g->dev);
+        double *sums = part + 3 * kStaticSlots;
+        if (g->pform) g->cell<true, OP_REDUCE>(g->f[g->cur], g->f[1 - g->cur], nullptr, part);
+        else g->cell<false, OP_REDUCE>(g->f[g->cur], nullptr, nullptr, part);
+        check(g->last_partials == blocks, "reduction pass shape mismatch");
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, g->stream, part, blocks, sums);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, sums, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
         HIPCHK(hipStreamSynchronize(g->stream));
     });
     if (rc == LBM_OK && !(std::isfinite(out[0]) && std::isfinite(out[1]) && std::isfinite(out[2])))
