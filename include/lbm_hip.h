@@ -147,7 +147,7 @@ int lbm_sync(lbm_grid *g);
 /* Step-kernel variants (no reference counterpart): how the fused step moves
  * memory -- cells per thread, non-temporal access, block order, or (8, 9) two
  * steps per launch through LDS, which reads and writes the state once per two
- * steps.  All variants compute the same bits.  The two-step variants need two
+ * steps (row blocks of 8 or 16).  All variants compute the same bits.  The two-step variants need two
  * more state sets: lbm_set_variant allocates them when it selects one (and
  * fails, changing nothing, when it cannot) and frees them when it selects
  * another.  lbm_autotune times each variant for `steps` steps on the grid's

@@ -157,8 +157,8 @@ __device__ __forceinline__ Moments collide_shifted(float g[9], float omega) {
 // flow is close to the free stream the stored values are then small, and
 // binary16's relative rounding error costs proportionally less velocity.  The
 // arithmetic stays in the variables of f - w: a value is shifted by
-// d_k = s_k - w_k when it is loaded and back before it is rounded.  on = 0
-// (and d = 0) is the plain f - w format.
+// d_k = s_k - w_k when it is loaded and back before it is rounded (StoreF16E).
+// on = 0 (and d = 0) describes the plain f - w format (StoreF16S).
 //
 // d_k is rounded to a multiple of 2^-20.  Every binary16 value is a multiple of
 // 2^-24, so for stored values below 16 in magnitude the f32 sum h + d_k (and
@@ -186,6 +186,7 @@ inline F16Shift make_f16_shift(bool on, double us) {
 struct StoreF32 {
     using T = float;
     static constexpr int bytes = 4;
+    static constexpr bool shifted = false;
     __device__ __forceinline__ static float load(T v, int, const F16Shift &) { return v; }
     __device__ __forceinline__ static float load_b(T v, int, bool, const F16Shift &) { return v; }
     __device__ __forceinline__ static T store(float w, int, const F16Shift &) { return w; }
@@ -200,28 +201,21 @@ struct StoreF32 {
 struct StoreF16S {
     using T = uint16_t;
     static constexpr int bytes = 2;
+    static constexpr bool shifted = false;
     // load is pinned like store: every kernel shape then performs exactly one
     // f16 -> f32 conversion per value, never a mixed-precision FMA in its place.
-    __device__ __forceinline__ static float load(T v, int k, const F16Shift &sh) {
+    __device__ __forceinline__ static float load(T v, int, const F16Shift &) {
         float g = (float)__builtin_bit_cast(_Float16, v);
         asm("" : "+v"(g));
-        if (sh.on) g += sh.d[k];
         return g;
     }
-    // population k's working value from a stored value of population opp(k) when
-    // `bounced` (half-way bounce-back hands a cell its own opposite population)
-    __device__ __forceinline__ static float load_b(T v, int k, bool bounced, const F16Shift &sh) {
-        float g = (float)__builtin_bit_cast(_Float16, v);
-        asm("" : "+v"(g));
-        if (sh.on) g += bounced ? sh.d[opp(k)] : sh.d[k];
-        return g;
-    }
+    // (f - w has the same shift for a population and its opposite)
+    __device__ __forceinline__ static float load_b(T v, int k, bool, const F16Shift &sh) { return load(v, k, sh); }
     // The empty asm pins g as an f32 register value, so the conversion is always
     // a separate f32 -> f16 round to nearest even; otherwise the compiler may fuse
     // it with the producing FMA (one rounding instead of two) in some kernel
     // shapes and not others, and kernels would disagree in the last bit.
-    __device__ __forceinline__ static T store(float g, int k, const F16Shift &sh) {
-        if (sh.on) g -= sh.d[k];
+    __device__ __forceinline__ static T store(float g, int, const F16Shift &) {
         asm("" : "+v"(g));
         return __builtin_bit_cast(uint16_t, (_Float16)g);
     }
@@ -232,6 +226,28 @@ struct StoreF16S {
     __device__ __forceinline__ static Moments collide(float w[9], float omega) { return collide_shifted(w, omega); }
     __device__ __forceinline__ static Moments moments(const float w[9], bool with_rest) {
         return moments_shifted(w, with_rest);
+    }
+};
+
+// Binary16 of f_k - s_k, s_k the free-stream equilibrium (F16Shift above): the
+// working variables are those of StoreF16S, shifted by d_k on the way in and
+// out (exact in f32 for stored values below 16 in magnitude).
+struct StoreF16E : StoreF16S {
+    static constexpr bool shifted = true;
+    __device__ __forceinline__ static float load(T v, int k, const F16Shift &sh) {
+        return StoreF16S::load(v, k, sh) + sh.d[k];
+    }
+    // population k's working value from a stored value of population opp(k) when
+    // `bounced` (half-way bounce-back hands a cell its own opposite population)
+    __device__ __forceinline__ static float load_b(T v, int k, bool bounced, const F16Shift &sh) {
+        return StoreF16S::load(v, k, sh) + (bounced ? sh.d[opp(k)] : sh.d[k]);
+    }
+    __device__ __forceinline__ static T store(float g, int k, const F16Shift &sh) {
+        return StoreF16S::store(g - sh.d[k], k, sh);
+    }
+    __device__ __forceinline__ static float to_f(T v, int k, const F16Shift &sh) { return load(v, k, sh) + weight(k); }
+    __device__ __forceinline__ static T from_f(float f, int k, const F16Shift &sh) {
+        return store(f - weight(k), k, sh);
     }
 };
 

@@ -277,13 +277,10 @@ struct lbm_batch {
 
     template <class F>
     void dispatch(F &&fn) {
-        if (precision == LBM_PRECISION_F32) {
-            if (boundary == LBM_BOUNDARY_REFERENCE) fn.template operator()<StoreF32, LBM_BOUNDARY_REFERENCE>();
-            else fn.template operator()<StoreF32, LBM_BOUNDARY_ZERO_GRADIENT>();
-        } else {
-            if (boundary == LBM_BOUNDARY_REFERENCE) fn.template operator()<StoreF16S, LBM_BOUNDARY_REFERENCE>();
-            else fn.template operator()<StoreF16S, LBM_BOUNDARY_ZERO_GRADIENT>();
-        }
+        by_store(precision, [&]<class St>() {
+            if (boundary == LBM_BOUNDARY_REFERENCE) fn.template operator()<St, LBM_BOUNDARY_REFERENCE>();
+            else fn.template operator()<St, LBM_BOUNDARY_ZERO_GRADIENT>();
+        });
     }
 
     void flush() {
@@ -490,12 +487,10 @@ int lbm_batch_reset(lbm_batch *b) {
             const float4 c = make_float4(v[4], v[5], v[6], v[7]);
             const dim3 grid((unsigned)(((size_t)d.g.ys * d.g.Y + 255) / 256));
             for (int s = 0; s < 2; ++s) {
-                if (b->precision == LBM_PRECISION_F32)
-                    hipLaunchKernelGGL(k_fill<StoreF32>, grid, dim3(256), 0, b->stream, (float *)b->fptr(s, d), d.g, a,
-                                       c, v[8]);
-                else
-                    hipLaunchKernelGGL(k_fill<StoreF16S>, grid, dim3(256), 0, b->stream, (uint16_t *)b->fptr(s, d),
-                                       d.g, a, c, v[8]);
+                by_store(b->precision, [&]<class St>() {
+                    hipLaunchKernelGGL(k_fill<St>, grid, dim3(256), 0, b->stream, (typename St::T *)b->fptr(s, d), d.g,
+                                       a, c, v[8]);
+                });
                 HIPCHK(hipGetLastError());
             }
         }
@@ -640,12 +635,10 @@ int lbm_batch_read_f(lbm_batch *b, uint32_t d, float *f9) {
         const dim3 grid((dd.g.X + 255) / 256, dd.g.Y);
         for (int k = 0; k < 9; ++k) {
             const char *plane = b->fptr(b->cur, dd) + (size_t)k * dd.g.ks * b->esize;
-            if (b->precision == LBM_PRECISION_F32)
-                hipLaunchKernelGGL(k_decode_plane<StoreF32>, grid, dim3(256), 0, b->stream, (const float *)plane,
+            by_store(b->precision, [&]<class St>() {
+                hipLaunchKernelGGL(k_decode_plane<St>, grid, dim3(256), 0, b->stream, (const typename St::T *)plane,
                                    b->dense, dd.g, k);
-            else
-                hipLaunchKernelGGL(k_decode_plane<StoreF16S>, grid, dim3(256), 0, b->stream,
-                                   (const uint16_t *)plane, b->dense, dd.g, k);
+            });
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(f9 + k * dn, b->dense, dn * sizeof(float), hipMemcpyDeviceToHost, b->stream));
         }

@@ -103,6 +103,14 @@ __global__ void k_build_segments(const uint8_t *__restrict__ flags, uint8_t *__r
     if (list && !clear && 64 * s < g.X) list[atomicAdd(count, 1u)] = (uint32_t)((size_t)y * srow + s);
 }
 
+// fn.template operator()<St>() with the storage format of `precision`
+template <class F>
+void by_store(int precision, F &&fn) {
+    if (precision == LBM_PRECISION_F32) fn.template operator()<StoreF32>();
+    else if (precision == LBM_PRECISION_F16E) fn.template operator()<StoreF16E>();
+    else fn.template operator()<StoreF16S>();
+}
+
 enum : int {
     OP_COLLIDE = 1,   // collide the gathered state and store all nine planes to dst
     OP_STORE_S = 2,   // store the gathered (streamed) state to dst
@@ -430,7 +438,7 @@ __device__ __forceinline__ void patched_tile(const typename St::T *src, typename
             const uint32_t fl = (uint32_t)(fw >> (8 * j)) & 0xFFu;
             const size_t sown = sown0 + j;
             if (kGatherEdges && edge) {
-                if constexpr (St::bytes == 2) {
+                if constexpr (St::shifted) {
                     gather_cell<St, MODE, true>(src, dst, flags, mask, g, x, y, own0 + j, sown, fe);
                     gathered = true;
                 } else {
@@ -466,7 +474,7 @@ __device__ __forceinline__ void patched_tile(const typename St::T *src, typename
         float f[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            if constexpr (kGatherEdges && St::bytes == 2) {
+            if constexpr (kGatherEdges && St::shifted) {
                 if (gathered) {  // no store/load round trip: the shifted format does not invert exactly
                     f[k] = fe[k];
                     continue;
@@ -474,7 +482,7 @@ __device__ __forceinline__ void patched_tile(const typename St::T *src, typename
             }
             f[k] = St::load_b(get_lane<CPT>(v[k], j), k, (from_opp[j] >> k) & 1u, g.sh);
         }
-        if constexpr ((OP & OP_STORE_S) && St::bytes == 2) {
+        if constexpr ((OP & OP_STORE_S) && St::shifted) {
             // the pulled state as stored values of each population: bounced
             // values are re-encoded with their new population's shift (values
             // that were not moved between populations come back bit for bit)

@@ -515,7 +515,9 @@ struct lbm_grid {
 
     // Variants 8, 9: two P-form steps per launch (k_pair), blocks of 8 or 16 rows
     // (measured at 8192 x 4096: 2 / 4 / 8 / 16 / 32 rows 362 / 312 / 303 / 314 /
-    // 333 us per step).
+    // 333 us per step).  Two cells per thread (pairs through the ring, 510-column
+    // strips) measured slower: f32 408 vs 297 us per step (64 KB of ring, 2 waves
+    // per SIMD), binary16 303 vs 255 us.
     static constexpr int kPairFirst = 8;
     static bool pair_variant(int v) { return v >= kPairFirst; }
     int pair_rows() const { return 8 << (variant - kPairFirst); }
@@ -575,28 +577,20 @@ struct lbm_grid {
     // two P-form steps in one launch; the other set is then whole only on the
     // cells the stream does not write
     void pair() {
-        if (precision == LBM_PRECISION_F32) {
-            if (boundary == LBM_BOUNDARY_REFERENCE) launch_pair<StoreF32, LBM_BOUNDARY_REFERENCE>();
-            else launch_pair<StoreF32, LBM_BOUNDARY_ZERO_GRADIENT>();
-        } else {
-            if (boundary == LBM_BOUNDARY_REFERENCE) launch_pair<StoreF16S, LBM_BOUNDARY_REFERENCE>();
-            else launch_pair<StoreF16S, LBM_BOUNDARY_ZERO_GRADIENT>();
-        }
+        by_store(precision, [&]<class St>() {
+            if (boundary == LBM_BOUNDARY_REFERENCE) launch_pair<St, LBM_BOUNDARY_REFERENCE>();
+            else launch_pair<St, LBM_BOUNDARY_ZERO_GRADIENT>();
+        });
     }
 
     template <bool PFORM, int OP>
     void cell(const void *src, void *dst, float *mom, double *part) {
-        if (precision == LBM_PRECISION_F32) {
+        by_store(precision, [&]<class St>() {
             if (boundary == LBM_BOUNDARY_REFERENCE)
-                launch_cell<StoreF32, LBM_BOUNDARY_REFERENCE, PFORM, OP>(src, dst, mom, part);
+                launch_cell<St, LBM_BOUNDARY_REFERENCE, PFORM, OP>(src, dst, mom, part);
             else
-                launch_cell<StoreF32, LBM_BOUNDARY_ZERO_GRADIENT, PFORM, OP>(src, dst, mom, part);
-        } else {
-            if (boundary == LBM_BOUNDARY_REFERENCE)
-                launch_cell<StoreF16S, LBM_BOUNDARY_REFERENCE, PFORM, OP>(src, dst, mom, part);
-            else
-                launch_cell<StoreF16S, LBM_BOUNDARY_ZERO_GRADIENT, PFORM, OP>(src, dst, mom, part);
-        }
+                launch_cell<St, LBM_BOUNDARY_ZERO_GRADIENT, PFORM, OP>(src, dst, mom, part);
+        });
     }
 
     void build_flags() {
@@ -614,11 +608,9 @@ struct lbm_grid {
         const float4 b = make_float4(v[4], v[5], v[6], v[7]);
         const dim3 grid((unsigned)(((size_t)geom.ys * Y + 255) / 256));
         for (int s = 0; s < 2; ++s) {
-            if (precision == LBM_PRECISION_F32)
-                hipLaunchKernelGGL(k_fill<StoreF32>, grid, dim3(256), 0, stream, (float *)f[s], geom, a, b, v[8]);
-            else
-                hipLaunchKernelGGL(k_fill<StoreF16S>, grid, dim3(256), 0, stream, (uint16_t *)f[s], geom, a, b,
-                                   v[8]);
+            by_store(precision, [&]<class St>() {
+                hipLaunchKernelGGL(k_fill<St>, grid, dim3(256), 0, stream, (typename St::T *)f[s], geom, a, b, v[8]);
+            });
             HIPCHK(hipGetLastError());
         }
     }
@@ -691,12 +683,10 @@ struct lbm_grid {
         // decode one pitched population plane into a dense f32 host plane
         ensure_dense();
         dim3 grid((X + 255) / 256, Y);
-        if (precision == LBM_PRECISION_F32)
-            hipLaunchKernelGGL(k_decode_plane<StoreF32>, grid, dim3(256), 0, stream, (const float *)dev_plane, dense,
-                               geom, k);
-        else
-            hipLaunchKernelGGL(k_decode_plane<StoreF16S>, grid, dim3(256), 0, stream, (const uint16_t *)dev_plane,
+        by_store(precision, [&]<class St>() {
+            hipLaunchKernelGGL(k_decode_plane<St>, grid, dim3(256), 0, stream, (const typename St::T *)dev_plane,
                                dense, geom, k);
+        });
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(host, dense, sizeof(float) * X * Y, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
@@ -706,12 +696,10 @@ struct lbm_grid {
         ensure_dense();
         HIPCHK(hipMemcpyAsync(dense, host, sizeof(float) * X * Y, hipMemcpyHostToDevice, stream));
         dim3 grid((X + 255) / 256, Y);
-        if (precision == LBM_PRECISION_F32)
-            hipLaunchKernelGGL(k_encode_plane<StoreF32>, grid, dim3(256), 0, stream, dense, (float *)dev_plane, geom,
-                               k);
-        else
-            hipLaunchKernelGGL(k_encode_plane<StoreF16S>, grid, dim3(256), 0, stream, dense, (uint16_t *)dev_plane,
+        by_store(precision, [&]<class St>() {
+            hipLaunchKernelGGL(k_encode_plane<St>, grid, dim3(256), 0, stream, dense, (typename St::T *)dev_plane,
                                geom, k);
+        });
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream));
     }
@@ -914,12 +902,10 @@ int lbm_single_cell(lbm_grid *g, int k) {
         }
         for (int s = 0; s < 2; ++s) {
             char *p = (char *)g->f[s] + g->state_offset(k, x, y);
-            if (g->precision == LBM_PRECISION_F32)
-                hipLaunchKernelGGL(k_set_one<StoreF32>, dim3(1), dim3(64), 0, g->stream, (float *)p, 4.0f, k,
+            by_store(g->precision, [&]<class St>() {
+                hipLaunchKernelGGL(k_set_one<St>, dim3(1), dim3(64), 0, g->stream, (typename St::T *)p, 4.0f, k,
                                    g->geom.sh);
-            else
-                hipLaunchKernelGGL(k_set_one<StoreF16S>, dim3(1), dim3(64), 0, g->stream, (uint16_t *)p, 4.0f, k,
-                                   g->geom.sh);
+            });
             HIPCHK(hipGetLastError());
         }
     });
@@ -931,11 +917,9 @@ int lbm_set_pulse(lbm_grid *g, int k, uint32_t x, uint32_t y, float v) {
         check(x < g->X && y < g->Y, "cell outside the grid");
         g->flush();
         char *p = (char *)g->f[g->cur] + g->state_offset(k, x, y);
-        if (g->precision == LBM_PRECISION_F32)
-            hipLaunchKernelGGL(k_set_one<StoreF32>, dim3(1), dim3(64), 0, g->stream, (float *)p, v, k, g->geom.sh);
-        else
-            hipLaunchKernelGGL(k_set_one<StoreF16S>, dim3(1), dim3(64), 0, g->stream, (uint16_t *)p, v, k,
-                               g->geom.sh);
+        by_store(g->precision, [&]<class St>() {
+            hipLaunchKernelGGL(k_set_one<St>, dim3(1), dim3(64), 0, g->stream, (typename St::T *)p, v, k, g->geom.sh);
+        });
         HIPCHK(hipGetLastError());
     });
 }
@@ -1224,17 +1208,15 @@ int lbm_surrogate_jump(lbm_grid *g, lbm_unet *net, uint32_t steps, float s_rho, 
         unet_forward_on(net, g->stream);
         const dim3 grid((g->X + 255) / 256, g->Y);
         const float *o = unet_output_tensor(net);
-#define APPLY(ST, TT, F)                                                                                          \
-    hipLaunchKernelGGL((k_apply_jump<ST, F>), grid, dim3(256), 0, g->stream, o, 32, g->now, g->mask, (TT *)g->f[g->cur], \
-                       g->geom, d_rho, d_u)
-        if (g->precision == LBM_PRECISION_F32) {
-            if (fine) APPLY(StoreF32, float, 1);
-            else APPLY(StoreF32, float, 2);
-        } else {
-            if (fine) APPLY(StoreF16S, uint16_t, 1);
-            else APPLY(StoreF16S, uint16_t, 2);
-        }
-#undef APPLY
+        by_store(g->precision, [&]<class St>() {
+            using T = typename St::T;
+            if (fine)
+                hipLaunchKernelGGL((k_apply_jump<St, 1>), grid, dim3(256), 0, g->stream, o, 32, g->now, g->mask,
+                                   (T *)g->f[g->cur], g->geom, d_rho, d_u);
+            else
+                hipLaunchKernelGGL((k_apply_jump<St, 2>), grid, dim3(256), 0, g->stream, o, 32, g->now, g->mask,
+                                   (T *)g->f[g->cur], g->geom, d_rho, d_u);
+        });
         HIPCHK(hipGetLastError());
         unet_end_use(net, g->stream);
         g->steps += steps;

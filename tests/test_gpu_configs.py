@@ -52,7 +52,7 @@ def _compare_sets(a, b, what):
         assert same_bits(ma, mb), f"{what}: renderer moments differ"
 
 
-@pytest.mark.parametrize("precision,variants", [(0, (8, 9, 1, 3)), (1, (4, 7, 8))])
+@pytest.mark.parametrize("precision,variants", [(0, (8, 9, 1, 3)), (1, (4, 7, 8)), (2, (4, 8))])
 def test_config2_8192x4096_kernel_shapes_equal_one_step(precision, variants):
     """8192 x 4096: 33 strips x 512 row blocks with the XCD-aware block order
     for the two-step kernel (about 16.9 k blocks), the several-cells-per-thread

@@ -458,7 +458,8 @@ def test_two_step_variants_whole_state_and_paint_between_calls(variant, precisio
     b.close()
 
 
-@pytest.mark.parametrize("X,Y", [(3, 3), (5, 4), (64, 9), (254, 17), (255, 8), (256, 3), (509, 12), (512, 25)])
+@pytest.mark.parametrize("X,Y", [(3, 3), (5, 4), (64, 9), (254, 17), (255, 8), (256, 3), (509, 12), (512, 25),
+                                 (507, 10), (508, 9), (510, 19), (1017, 7), (1018, 33)])
 def test_two_step_variants_small_and_boundary_sizes(X, Y):
     """Grids at and around the strip width (254 columns), the row block (8 or 16
     rows) and the row pitch (a multiple of 64, where the east pull is clamped at

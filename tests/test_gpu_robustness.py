@@ -89,7 +89,7 @@ def test_autotune_with_few_steps_skips_two_step_variants():
     b.step(4)
     best, ms = a.autotune(2)
     assert best < 8
-    assert np.isinf(ms[8]) and np.isinf(ms[9])
+    assert all(np.isinf(ms[v]) for v in range(8, len(ms)))
     assert np.all(np.isfinite(ms[:8]))
     b.variant = best
     a.step(6)
