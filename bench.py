@@ -91,21 +91,26 @@ def surrogate_leg(steps=10000):
 
     12 seeded circles (r 8..32, seed 2024), tau 0.6, u_in 0.05, 2000 plain
     spin-up steps; then `steps` steps of plain LBM and of the rollout (jump 64
-    with the half-resolution network, 8 corrector steps, residual gate 0.05)
-    from the same state.  Reported: the wall-clock speed-up (both sides end
-    with a stream sync), the velocity rel-L2 of the rollout against the plain
-    LBM at the end and after 1 and 10 cycles, and the same short-horizon
-    errors for the network-free ablation (the change predicted as zero:
-    re-equilibration + correctors only).  Both sides step with the autotuned
-    step kernel."""
+    with the half-resolution network, 8 corrector steps) from the same state.
+    Gated: every 8th cycle is a check that runs the cycle both ways and keeps
+    the plain result; a forecast error above half that of predicting no change
+    distrusts the network until the next check (fallback to plain).  Reported: the wall-clock speed-up
+    (both sides end with a stream sync) gated and ungated, the velocity rel-L2
+    against plain LBM at the end, the gate's residuals and decisions for the
+    trained network and for the network-free ablation (the change predicted as
+    zero: re-equilibration + correctors only), the short-horizon errors of
+    both ungated, and the method's own floor (a perfect prediction: the plain
+    LBM's 64 steps, then re-equilibration, then the correctors).  Both sides
+    step with the autotuned step kernel."""
     import numpy as np
 
     from lbm_amd import LBM, gen_circles
-    from lbm_amd.rollout import Rollout, compare, velocity_error
+    from lbm_amd.rollout import Rollout, compare, method_floor, velocity_error
     from lbm_amd.surrogate import UNet, load_params
     if not os.path.exists(SURROGATE_WEIGHTS):
         return {"skipped": f"no trained weights at {os.path.relpath(SURROGATE_WEIGHTS, ROOT)}"}
     X, Y, tau, u = 1024, 512, 0.6, 0.05
+    threshold, check_every = 0.5, 8
     mask = gen_circles(X, Y, SEED, 12, 8, 32)
 
     def make():
@@ -117,30 +122,55 @@ def surrogate_leg(steps=10000):
         return g
     net = UNet(1, Y // 2, X // 2, seed=1)
     params = load_params(SURROGATE_WEIGHTS)
+    zero = np.zeros_like(params)
     out = {"workload": f"{X}x{Y}, 12 circles r 8-32 (seed {SEED}), tau {tau}, u_in {u}, after 2000 steps; "
-                       f"jump 64 with a {X // 2}x{Y // 2} U-Net, 8 corrector steps, residual gate 0.05",
+                       f"jump 64 with a {X // 2}x{Y // 2} U-Net, 8 corrector steps; gate: a check every "
+                       f"{check_every} cycles, threshold {threshold} of the persistence forecast's error",
            "steps": steps}
     short = {}
     horizons = (1, 2, 3, 5, 10)
-    for label, p in (("trained", params), ("zero_change", np.zeros_like(params))):
+    for label, p in (("trained", params), ("zero_change", zero)):
         net.set_params(p)
         errs = {}
         for cycles in horizons:
             a, b = make(), make()
-            Rollout(a, net, threshold=float("inf")).run(72 * cycles)
+            Rollout(a, net, check_every=0).run(72 * cycles)
             b.step(72 * cycles)
             errs[cycles] = round(velocity_error(a.fields(), b.fields(), mask), 5)
             a.close()
             b.close()
         short[label] = {f"u_rel_l2_after_{c}_cycles": e for c, e in errs.items()}
         short[label]["cycles_within_1pct"] = max([c for c in horizons if errs[c] <= 0.01], default=0)
+    out["short_horizon_ungated"] = short
     net.set_params(params)
-    speedup, err, st, t_plain, t_roll = compare(make, net, steps, threshold=0.05)
-    fwd_ms = net.time_forward(10)
+    sp, err, st, t_plain, t_roll = compare(make, net, steps, threshold=threshold, check_every=check_every)
+    out.update({"speedup": round(sp, 3), "u_rel_l2_at_end": round(err, 5), "jumps": st.jumps, "checks": st.checks,
+                "fallbacks": st.fallbacks, "residuals": [round(r, 4) for r in st.residuals],
+                "check_errors": [round(e, 5) for e in st.errors],
+                "plain_ms": round(t_plain * 1e3, 2), "rollout_ms": round(t_roll * 1e3, 2)})
+    sp0, err0, st0, _, t0 = compare(make, net, steps, check_every=0)
+    out["ungated"] = {"speedup": round(sp0, 3), "u_rel_l2_at_end": round(err0, 5), "jumps": st0.jumps,
+                      "rollout_ms": round(t0 * 1e3, 2)}
+    net.set_params(zero)
+    spz, errz, stz, _, _ = compare(make, net, steps, threshold=threshold, check_every=check_every)
+    out["ablation_gated"] = {"speedup": round(spz, 3), "u_rel_l2_at_end": round(errz, 5), "jumps": stz.jumps,
+                             "checks": stz.checks, "fallbacks": stz.fallbacks,
+                             "residuals": [round(r, 4) for r in stz.residuals[:4]],
+                             "check_errors": [round(e, 5) for e in stz.errors[:4]]}
+    floor = {}
+    a, b = make(), make()
+    done = 0
+    for cycles in (1, 10, steps // 72):
+        method_floor(a, net, 72 * cycles - done)  # zero parameters: re-equilibration only
+        b.step(72 * cycles - done)
+        done = 72 * cycles
+        floor[f"u_rel_l2_after_{cycles}_cycles"] = round(velocity_error(a.fields(), b.fields(), mask), 5)
+    a.close()
+    b.close()
+    out["method_floor"] = floor
+    net.set_params(params)
+    out["unet_forward_ms"] = round(net.time_forward(10), 4)
     net.close()
-    out.update({"speedup": round(speedup, 3), "u_rel_l2_at_end": round(err, 5), "jumps": st.jumps,
-                "fallbacks": st.fallbacks, "plain_ms": round(t_plain * 1e3, 2), "rollout_ms": round(t_roll * 1e3, 2),
-                "unet_forward_ms": round(fwd_ms, 4), "short_horizon": short})
     return out
 
 

@@ -342,7 +342,16 @@ int lbm_restore(lbm_grid *g);
  * rel is +inf when either field holds a NaN or Inf. */
 int lbm_velocity_mark(lbm_grid *g);
 int lbm_velocity_change(lbm_grid *g, double *rel);
-
+/* The rollout's check.  lbm_velocity_mark_slot(g, s) marks the current
+ * velocity field in slot s (0 or 1; lbm_velocity_mark is slot 0).  With slot 0
+ * holding the field at a cycle's start and slot 1 the surrogate's forecast for
+ * its end, and the grid now at the plain LBM's end of the same cycle:
+ * out[0] = |u - m1| / |u - m0| (the forecast's error over that of persistence:
+ * 0 perfect, 1 no better than predicting no change), out[1] = |u - m1| / |m1|
+ * (its relative error), over fluid cells with FP64 sums on the device; +inf
+ * when a field holds a NaN or Inf. */
+int lbm_velocity_mark_slot(lbm_grid *g, int slot);
+int lbm_velocity_skill(lbm_grid *g, double out[2]);
 /* The surrogate's 3x3 convolution primitives on host arrays, for testing: NHWC
  * f32 tensors rounded to bf16 on the device, f32 accumulation on the matrix
  * cores.  w is [9][Cout][Cin] (tap = 3 dy + dx).  mode 0: out (N,H,W,Cout) =

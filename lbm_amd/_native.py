@@ -134,6 +134,8 @@ SIGNATURES = {
     "lbm_restore": (_i32, [_P]),
     "lbm_velocity_mark": (_i32, [_P]),
     "lbm_velocity_change": (_i32, [_P, _dp]),
+    "lbm_velocity_mark_slot": (_i32, [_P, _i32]),
+    "lbm_velocity_skill": (_i32, [_P, _dp]),
     "lbm_conv3x3_test": (_i32, [_i32, _i32, _i32, _i32, _i32, _i32, _fp, _fp, _fp, _fp, _fp, _i32]),
     "lbm_f64_last_error": (ctypes.c_char_p, []),
     "lbm_f64_create": (_i32, [_u32, _u32, ctypes.c_double, ctypes.c_double, _i32, _i32, _PP]),

@@ -369,6 +369,32 @@ __global__ __launch_bounds__(kBlock) void k_velocity_change(const float *__restr
     block_sums(s0, s1, 0.0, partials, (size_t)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
+// The rollout's check: u the plain LBM's velocity after a cycle, m0 the field
+// at the cycle's start (the persistence forecast), m1 the surrogate's forecast;
+// over fluid cells s0 = |u - m1|^2, s1 = |u - m0|^2, s2 = |m1|^2.
+__global__ __launch_bounds__(kBlock) void k_velocity_skill(const float *__restrict__ mom,
+                                                           const float *__restrict__ m0,
+                                                           const float *__restrict__ m1,
+                                                           const uint8_t *__restrict__ mask, Geom g,
+                                                           double *__restrict__ partials) {
+    const int x = blockIdx.x * kBlock + threadIdx.x;
+    const int y = blockIdx.y;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (x < g.X) {
+        const size_t i = (size_t)y * g.pitch + x;
+        if (mask[i] != 1) {
+            const float rho = mom[2 * g.pstride + i];
+            const double ux = mom[i] / rho, uy = mom[g.pstride + i] / rho;
+            const double fx = m1[i], fy = m1[g.pstride + i];
+            const double px = m0[i], py = m0[g.pstride + i];
+            s0 = (ux - fx) * (ux - fx) + (uy - fy) * (uy - fy);
+            s1 = (ux - px) * (ux - px) + (uy - py) * (uy - py);
+            s2 = fx * fx + fy * fy;
+        }
+    }
+    block_sums(s0, s1, s2, partials, (size_t)blockIdx.y * gridDim.x + blockIdx.x);
+}
+
 __global__ void k_velocity_mark(const float *__restrict__ mom, float *__restrict__ mark, Geom g) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
@@ -430,7 +456,16 @@ struct lbm_grid {
     int snap_cur = 0;
     bool snap_pform = false;
     uint64_t snap_steps = 0;
-    float *umark = nullptr;  // velocity planes for the surrogate's residual
+    float *umark = nullptr;  // two slots of velocity planes for the surrogate's residuals
+    int umarked = 0;         // bit s: slot s holds a mark
+
+    void ensure_vpart(dim3 grid) {
+        if (!vpart) {
+            nvpart = (size_t)grid.x * grid.y;
+            vpart = dalloc<double>(3 * nvpart);
+        }
+        if (!sums) sums = dalloc<double>(3);
+    }
     double *vpart = nullptr;
     size_t nvpart = 0;
 
@@ -625,7 +660,7 @@ struct lbm_grid {
         if (!partials) {
             npartials = (size_t)shape(1).nblocks;  // the largest pass
             partials = dalloc<double>(3 * npartials);
-            sums = dalloc<double>(3);
+            if (!sums) sums = dalloc<double>(3);
         }
     }
 
@@ -1149,25 +1184,49 @@ int lbm_restore(lbm_grid *g) {
     });
 }
 
-int lbm_velocity_mark(lbm_grid *g) {
+int lbm_velocity_mark_slot(lbm_grid *g, int slot) {
     return guarded(g, [&] {
-        if (!g->umark) g->umark = dalloc<float>(2 * (size_t)g->geom.pstride);
+        check(slot == 0 || slot == 1, "slot must be 0 or 1");
+        if (!g->umark) g->umark = dalloc<float>(4 * (size_t)g->geom.pstride);  // two slots of (ux, uy)
         g->moments_now(false);
         hipLaunchKernelGGL(k_velocity_mark, dim3((g->X + 255) / 256, g->Y), dim3(256), 0, g->stream, g->now,
-                           g->umark, g->geom);
+                           g->umark + 2 * (size_t)slot * g->geom.pstride, g->geom);
         HIPCHK(hipGetLastError());
+        g->umarked |= 1 << slot;
+    });
+}
+
+int lbm_velocity_mark(lbm_grid *g) { return lbm_velocity_mark_slot(g, 0); }
+
+int lbm_velocity_skill(lbm_grid *g, double out[2]) {
+    return guarded(g, [&] {
+        check(out != nullptr && g->umarked == 3, "mark both velocity slots first");
+        const dim3 grid((g->X + kBlock - 1) / kBlock, g->Y);
+        g->ensure_vpart(grid);
+        g->moments_now(false);
+        hipLaunchKernelGGL(k_velocity_skill, grid, dim3(kBlock), 0, g->stream, g->now, g->umark,
+                           g->umark + 2 * (size_t)g->geom.pstride, g->mask, g->geom, g->vpart);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, g->stream, g->vpart, g->nvpart, g->sums);
+        HIPCHK(hipGetLastError());
+        double s[3];
+        HIPCHK(hipMemcpyAsync(s, g->sums, sizeof s, hipMemcpyDeviceToHost, g->stream));
+        HIPCHK(hipStreamSynchronize(g->stream));
+        auto ratio = [](double num, double den) {
+            if (!std::isfinite(num) || !std::isfinite(den)) return (double)INFINITY;
+            if (den > 0.0) return std::sqrt(num / den);
+            return num > 0.0 ? (double)INFINITY : 0.0;
+        };
+        out[0] = ratio(s[0], s[1]);  // forecast error over persistence error
+        out[1] = ratio(s[0], s[2]);  // forecast error relative to the forecast
     });
 }
 
 int lbm_velocity_change(lbm_grid *g, double *rel) {
     return guarded(g, [&] {
-        check(rel != nullptr && g->umark != nullptr, "call lbm_velocity_mark first");
+        check(rel != nullptr && (g->umarked & 1), "call lbm_velocity_mark first");
         const dim3 grid((g->X + kBlock - 1) / kBlock, g->Y);
-        if (!g->vpart) {
-            g->nvpart = (size_t)grid.x * grid.y;
-            g->vpart = dalloc<double>(3 * g->nvpart);
-            if (!g->sums) g->sums = dalloc<double>(3);
-        }
+        g->ensure_vpart(grid);
         g->moments_now(false);
         hipLaunchKernelGGL(k_velocity_change, grid, dim3(kBlock), 0, g->stream, g->now, g->umark, g->mask, g->geom,
                            g->vpart);

@@ -300,8 +300,15 @@ class LBM:
     def restore(self):
         self._chk(self._lib.lbm_restore(self._h))
 
-    def velocity_mark(self):
-        self._chk(self._lib.lbm_velocity_mark(self._h))
+    def velocity_mark(self, slot: int = 0):
+        self._chk(self._lib.lbm_velocity_mark_slot(self._h, slot))
+
+    def velocity_skill(self):
+        """(|u - m1| / |u - m0|, |u - m1| / |m1|) over fluid cells: u the current
+        velocity, m0 / m1 the marks of slots 0 and 1."""
+        out = np.zeros(2, np.float64)
+        self._chk(self._lib.lbm_velocity_skill(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return float(out[0]), float(out[1])
 
     def velocity_change(self) -> float:
         r = ctypes.c_double()

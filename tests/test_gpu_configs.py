@@ -250,7 +250,7 @@ def test_config5_rollout_three_cycles():
     for label, p in (("trained", params), ("zero", np.zeros_like(params))):
         net.set_params(p)
         a, b = make(), make()
-        st = Rollout(a, net, threshold=float("inf")).run(3 * 72)
+        st = Rollout(a, net, check_every=0).run(3 * 72)
         assert st.jumps == 3 and st.fallbacks == 0
         b.step(3 * 72)
         errs[label] = velocity_error(a.fields(), b.fields(), mask)

@@ -201,7 +201,9 @@ def test_training_reduces_loss():
 
 @pytest.mark.gpu
 def test_rollout_forced_fallback_is_plain_lbm():
-    """threshold 0: every cycle falls back, so the rollout equals the plain LBM bit for bit."""
+    """threshold 0: the first cycle's check fails (its plain result is kept) and
+    every later cycle runs plain, so the rollout equals the plain LBM bit for
+    bit."""
     from lbm_amd import LBM, gen_circles
     from lbm_amd.rollout import Rollout
     from lbm_amd.surrogate import UNet
@@ -218,7 +220,7 @@ def test_rollout_forced_fallback_is_plain_lbm():
     a, b = make(), make()
     st = Rollout(a, net, threshold=0.0).run(72 * 3 + 10)
     b.step(72 * 3 + 10)
-    assert st.fallbacks == 3 and st.jumps == 0
+    assert st.fallbacks == 3 and st.jumps == 0 and st.checks == 1
     assert a.get_compute_num() == b.get_compute_num()
     assert np.array_equal(a.read_f().view(np.uint32), b.read_f().view(np.uint32))
 
@@ -239,11 +241,11 @@ def test_rollout_jump_path():
         g.step(200)
         return g
     a, b = make(), make()
-    st = Rollout(a, net, threshold=float("inf")).run(72 * 4)
+    st = Rollout(a, net, check_every=0).run(72 * 4)
     b.step(72 * 4)
     assert st.jumps == 4 and st.fallbacks == 0 and a.get_compute_num() == b.get_compute_num()
     err = velocity_error(a.fields(), b.fields(), mask)
-    print(f"untrained-network rollout error vs plain LBM: {err:.3e}; residuals {st.residuals}")
+    print(f"untrained-network rollout error vs plain LBM: {err:.3e}")
     assert np.isfinite(err) and err < 1.0
 
 
@@ -336,7 +338,7 @@ def test_rollout_coarse_network_jump_path():
         g.step(200)
         return g
     a, b = make(), make()
-    st = Rollout(a, net, threshold=float("inf")).run(72 * 3)
+    st = Rollout(a, net, check_every=0).run(72 * 3)
     b.step(72 * 3)
     assert st.jumps == 3 and a.get_compute_num() == b.get_compute_num()
     err = velocity_error(a.fields(), b.fields(), mask)
@@ -370,7 +372,7 @@ def test_trained_network_beats_the_network_free_method():
             g.step(1500)
             return g
         a, b = make(), make()
-        Rollout(a, net, threshold=float("inf")).run(72)
+        Rollout(a, net, check_every=0).run(72)
         b.step(72)
         errs[label] = velocity_error(a.fields(), b.fields(), mask)
         a.close()
@@ -378,6 +380,84 @@ def test_trained_network_beats_the_network_free_method():
     net.close()
     print(f"one-cycle velocity error: trained {errs['trained']:.4e}, zero-change {errs['zero']:.4e}")
     assert errs["trained"] < 0.5 * errs["zero"]
+
+
+@pytest.mark.gpu
+def test_gate_keeps_the_trained_network_and_rejects_the_network_free_method():
+    """The plain LBM judges the error: on an unseen geometry a check cycle
+    measures the surrogate cycle's true velocity error against the plain LBM,
+    as a fraction of the error of predicting no change.  With the default
+    threshold (0.5) the trained network passes every check and jumps between
+    them; the network-free method (change predicted as zero) scores about 1
+    (0.94: the correctors move it a little), fails
+    every check and every cycle runs plain, so its rollout equals the plain
+    LBM bit for bit."""
+    import os
+    from lbm_amd import LBM, gen_circles
+    from lbm_amd.rollout import Rollout, velocity_error
+    from lbm_amd.surrogate import UNet, load_params
+    params = load_params(os.path.join(os.path.dirname(__file__), "..", "lbm_amd", "data", "surrogate_f2.npy"))
+    X, Y = 1024, 256
+    mask = gen_circles(X, Y, 99, 8, 8, 24)
+    net = UNet(1, Y // 2, X // 2, seed=1)
+
+    def make():
+        g = LBM(1 / 0.65, X, Y)
+        g.mask = mask
+        g.custom_speed(0.06)
+        g.step(1500)
+        return g
+    out = {}
+    for label, p in (("trained", params), ("zero", np.zeros_like(params))):
+        net.set_params(p)
+        a, b = make(), make()
+        st = Rollout(a, net, check_every=4).run(72 * 8)
+        b.step(72 * 8)
+        out[label] = (st, velocity_error(a.fields(), b.fields(), mask), a.read_f(), b.read_f())
+        a.close()
+        b.close()
+    net.close()
+    st, err, _, _ = out["trained"]
+    print(f"trained: residuals {st.residuals}, jumps {st.jumps}, error {err:.4f}")
+    assert st.checks == 2 and st.fallbacks == 0 and st.jumps == 6
+    assert all(r < 0.3 for r in st.residuals) and all(e < 0.01 for e in st.errors) and err < 0.02
+    st, err, fa, fb = out["zero"]
+    print(f"zero change: residuals {st.residuals}, fallbacks {st.fallbacks}")
+    assert st.checks == 2 and st.jumps == 0 and st.fallbacks == 8
+    assert all(r > 0.8 for r in st.residuals)  # no change predicted: close to persistence
+    assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_method_floor_over_ten_cycles():
+    """The rollout with a perfect prediction (the plain LBM's own 64 steps,
+    then re-equilibration, then 8 correctors) on the benchmark's flow stays
+    within 0.5% of plain LBM over ten cycles: the error budget left to the
+    network."""
+    from lbm_amd import LBM, gen_circles
+    from lbm_amd.rollout import method_floor, velocity_error
+    from lbm_amd.surrogate import UNet
+    X, Y = 1024, 512
+    mask = gen_circles(X, Y, 2024, 12, 8, 32)
+
+    def make():
+        g = LBM(1 / 0.6, X, Y)
+        g.mask = mask
+        g.custom_speed(0.05)
+        g.step(2000)
+        return g
+    net = UNet(1, Y // 2, X // 2, seed=1)
+    net.set_params(np.zeros(net.num_params, np.float32))
+    a, b = make(), make()
+    assert method_floor(a, net, 720) == 10
+    b.step(720)
+    assert a.get_compute_num() == b.get_compute_num()
+    err = velocity_error(a.fields(), b.fields(), mask)
+    print(f"method floor after 10 cycles: {err:.5f}")
+    assert 0 < err < 0.005
+    net.close()
+    a.close()
+    b.close()
 
 
 @pytest.mark.gpu
@@ -400,7 +480,7 @@ def test_rollout_graph_replay_equals_plain_launches():
         g.mask = mask
         g.custom_speed(0.05)
         g.step(300)
-        Rollout(g, net, threshold=float("inf")).run(72 * 4)
+        Rollout(g, net, check_every=0).run(72 * 4)
         out.append(g.read_f())
         g.close()
         net.close()
