@@ -11,13 +11,16 @@ before the timed region; K steps run between two stream synchronisations.
 Printed: ONE JSON line with the FP32 MLUPS as `value`, the roofline of the
 fused step kernel (72 algorithmic bytes per cell update times the updates one
 launch makes -- two for the two-step variants, which read and write the state
-once per pair of steps -- over the average launch duration from HIP events
-recorded on the grid's own stream), the same for FP16-shifted storage (36 B
-per cell update), both again at 4096 x 2048 and 16384 x 8192 (grid_leg), the
-batched configuration's MLUPS (batched_leg, 5000 steps), the surrogate
-rollout's speed-up and velocity error against plain FP32 LBM (surrogate_leg),
-and the CPU baseline: the oracle's faithful f32 restatement of the reference's
-eight passes, timed on a bounded sample of the same workload on the host cores.
+once per pair of steps -- over the average launch duration of that kernel
+from HIP events recorded on the grid's own stream; `frac_hbm` the same with
+the launch's measured HBM bytes, calibrated FETCH_SIZE + WRITE_SIZE), the same
+for FP16-shifted storage (36 B per cell update; f - w, and centred on the
+free-stream equilibrium), both again at 4096 x 2048 and 16384 x 8192
+(grid_leg), the batched configuration's MLUPS (batched_leg, 5000 steps), the
+surrogate rollout's speed-up and velocity error against plain FP32 LBM
+(surrogate_leg), and the CPU baseline: the oracle's faithful f32 restatement
+of the reference's eight passes built for the host, timed on a bounded sample
+of the same workload on all host cores and on one.
 """
 from __future__ import annotations
 
@@ -50,6 +53,10 @@ def read_traffic(name):
     return float(m.group(1)) if m else None
 
 
+KERNEL_NAMES = {1: "k_cell (one step per launch)",
+                2: "k_pair + k_pair_col0 (two steps per launch: the state read and written once per pair of steps)"}
+
+
 def updates_per_launch(variant):
     """Cell updates per cell in one launch of the step kernel: variants 8 and 9
     run two steps per launch (the state is read and written once per pair)."""
@@ -79,8 +86,22 @@ def run_gpu(precision, X, Y, mask, steps, warmup, omega, u_in, variant=None):
     except LbmError as e:  # the flow left BGK's stable range: timing stands, sums do not
         log(f"conservation: {e}")
         cons = None
+    tuned["kernel_us"] = launch_us(g, tuned["variant"])
     g.close()
     return wall, dev_ms, cons, tuned
+
+
+def launch_us(g, variant, launches=40):
+    """Average duration of one launch of the step kernel, from HIP events on the
+    grid's stream.  A call of n steps with a two-step variant ends with one-step
+    launches, so the two-step launch is timed as the difference of a call of
+    2 + 2m steps and one of 2 steps (m two-step launches, each with its column
+    kernel, which it needs)."""
+    if updates_per_launch(variant) == 2:
+        g.step_timed(2)
+        short = min(g.step_timed(2) for _ in range(3))
+        return round((g.step_timed(2 + 2 * launches) - short) / launches * 1e3, 2)
+    return round(g.step_timed(launches) / launches * 1e3, 2)
 
 
 SURROGATE_WEIGHTS = os.path.join(ROOT, "lbm_amd", "data", "surrogate_f2.npy")
@@ -221,31 +242,57 @@ def batched_leg(n_domains=512, steps=5000):
     return out
 
 
+def _host_cpu():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(X, Y, mask, omega, u_in, budget_s):
+    """The oracle's faithful f32 restatement of the reference's eight passes,
+    built for this host (-O3 -march=native, no contraction; same bits as the
+    oracle), timed on the same workload on all host cores (the value) and on
+    one core; about budget_s seconds each after two warm-up steps."""
+    import tempfile
+
     import numpy as np
 
     import oracle
-    threads = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
-    r = oracle.RefCPU(X, Y, omega, threads=threads)
-    r.set_mask(mask.astype(np.uint32))
-    r.reset_equilibrium(1.0, u_in, 0.0)
-    r.step(1)
-    t0 = time.perf_counter()
-    r.step(1)
-    t1 = time.perf_counter() - t0
-    n = max(2, min(400, int(budget_s / max(t1, 1e-6))))
-    t0 = time.perf_counter()
-    r.step(n)
-    t = time.perf_counter() - t0
-    del r
+    lib = oracle.build_native(tempfile.mkdtemp(prefix="lbm_cpu_baseline_"))
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+
+    def sample(threads, budget):
+        r = oracle.RefCPU(X, Y, omega, threads=threads, lib_path=lib)
+        r.set_mask(mask.astype(np.uint32))
+        r.reset_equilibrium(1.0, u_in, 0.0)
+        r.step(1)
+        t0 = time.perf_counter()
+        r.step(1)
+        t1 = time.perf_counter() - t0
+        n = max(2, min(400, int(budget / max(t1, 1e-6))))
+        t0 = time.perf_counter()
+        r.step(n)
+        t = time.perf_counter() - t0
+        del r
+        return X * Y * n / t / 1e6, n, t
+
+    v_all, n_all, t_all = sample(cores, budget_s)
+    v_one, n_one, t_one = sample(1, budget_s / 2)
     return {
-        "value": round(X * Y * n / t / 1e6, 2),
+        "value": round(v_all, 2),
         "unit": "MLUPS",
-        "cores": threads,
+        "cores": cores,
         "kind": "port",
-        "sample": f"oracle refcpu (f32 restatement of the reference's 8 WGSL passes, OpenMP, one "
-                  f"parallel loop per pass) on the same {X}x{Y} workload, {n} steps after 2 warm-up "
-                  f"steps, {t:.1f} s wall",
+        "single_thread": round(v_one, 2),
+        "host": f"{_host_cpu()}, {os.cpu_count()} CPUs visible, {cores} in this process's affinity",
+        "build": "gcc -O3 -march=native -fopenmp -ffp-contract=off (bit for bit the oracle's build)",
+        "sample": f"oracle refcpu (f32 restatement of the reference's 8 WGSL passes, OpenMP, one parallel loop "
+                  f"per pass) on the same {X}x{Y} workload after 2 warm-up steps: {n_all} steps on {cores} "
+                  f"threads in {t_all:.1f} s; {n_one} steps on 1 thread in {t_one:.1f} s",
     }
 
 
@@ -285,8 +332,8 @@ def main():
     wall, dev_ms, cons, tuned = run_gpu(PRECISION_F32, X, Y, mask, args.steps, args.warmup, omega, u_in,
                                        args.variant)
     mlups = cells * args.steps / wall / 1e6
-    kern_s = dev_ms / 1e3 / args.steps
     upl = updates_per_launch(tuned["variant"])
+    kern_s = tuned["kernel_us"] / 1e6 / upl  # per cell update pass, from the dominant kernel's launches
     achieved = 72.0 * cells / kern_s / 1e9
     log(f"f32: {mlups:.0f} MLUPS, {kern_s * 1e6:.1f} us per step, {upl} step(s) per launch, "
         f"{achieved:.0f} GB/s algorithmic")
@@ -320,9 +367,13 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "kernel": KERNEL_NAMES[upl],
             "algorithmic_bytes_per_launch": 72 * cells * upl,
             "cell_updates_per_launch": cells * upl,
-            "kernel_avg_us": round(kern_s * upl * 1e6, 2),
+            "kernel_avg_us": tuned["kernel_us"],
+            # measured HBM bytes (calibrated FETCH_SIZE + WRITE_SIZE) per launch over the launch time
+            "achieved_hbm": round(traffic / (tuned["kernel_us"] * 1e-6) / 1e9, 1) if traffic else None,
+            "frac_hbm": round(traffic / (tuned["kernel_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
         },
         "fp64_mass_momentum": [float(c) for c in cons] if cons is not None else None,
         "autotune": tuned,
@@ -333,22 +384,39 @@ def main():
         w16, d16, c16, tuned16 = run_gpu(PRECISION_F16S, X, Y, mask, args.steps, args.warmup, omega,
                                         u_in, args.f16_variant)
         m16 = cells * args.steps / w16 / 1e6
-        k16 = d16 / 1e3 / args.steps
         u16 = updates_per_launch(tuned16["variant"])
+        k16 = tuned16["kernel_us"] / 1e6 / u16
         a16 = 36.0 * cells / k16 / 1e9
         log(f"f16s: {m16:.0f} MLUPS, {k16 * 1e6:.1f} us per step, {u16} step(s) per launch, "
             f"{a16:.0f} GB/s algorithmic")
+        t16 = read_traffic("traffic_f16s_8192x4096.txt") if (X, Y) == (8192, 4096) and u16 == 1 else None
         out["f16s"] = {
             "mlups": round(m16, 1),
             "speedup_vs_f32": round(m16 / mlups, 3),
             "roofline": {"bound": "hbm", "achieved": round(a16, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(a16 / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": 36 * cells * u16,
-                         "cell_updates_per_launch": cells * u16, "kernel_avg_us": round(k16 * u16 * 1e6, 2),
-                         "traffic": read_traffic("traffic_f16s_8192x4096.txt")
-                         if (X, Y) == (8192, 4096) and u16 == 1 else None},
+                         "frac": round(a16 / HBM_PEAK_GBS, 4), "kernel": KERNEL_NAMES[u16],
+                         "algorithmic_bytes_per_launch": 36 * cells * u16,
+                         "cell_updates_per_launch": cells * u16, "kernel_avg_us": tuned16["kernel_us"],
+                         "traffic": t16,
+                         "achieved_hbm": round(t16 / (tuned16["kernel_us"] * 1e-6) / 1e9, 1) if t16 else None,
+                         "frac_hbm": round(t16 / (tuned16["kernel_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+                         if t16 else None},
             "fp64_mass_momentum": [float(c) for c in c16] if c16 is not None else None,
             "autotune": tuned16,
         }
+        # against the FP32 one-step kernel (one pass over the state per step,
+        # as the binary16 kernel makes): the ratio of the storage formats alone
+        f32_one = min(tuned["us_per_step_by_variant"][:8]) if tuned.get("us_per_step_by_variant") else None
+        if f32_one:
+            out["f16s"]["speedup_vs_f32_one_step_kernel"] = round(f32_one / (k16 * 1e6), 3)
+        log("f16e: same workload, binary16 centred on the free-stream equilibrium")
+        from lbm_amd import PRECISION_F16E
+        w16e, _, c16e, tuned16e = run_gpu(PRECISION_F16E, X, Y, mask, args.steps, args.warmup, omega, u_in,
+                                          args.f16_variant)
+        m16e = cells * args.steps / w16e / 1e6
+        out["f16e"] = {"mlups": round(m16e, 1), "speedup_vs_f32": round(m16e / mlups, 3),
+                       "kernel_avg_us": tuned16e["kernel_us"], "variant": tuned16e["variant"],
+                       "fp64_mass_momentum": [float(c) for c in c16e] if c16e is not None else None}
 
     if not args.no_large and not args.no_f16:
         log("4096x2048 and 16384x8192: f32 and f16s")

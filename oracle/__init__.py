@@ -41,13 +41,31 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
-def _load():
+def build_native(out_dir: str) -> str:
+    """refcpu.c + oracle64.c built for this host (-O3 -march=native, no
+    contraction) into out_dir/liboracle_native.so: the CPU baseline's build."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "native", f"NATIVE_OUT={out_dir}"], check=True)
+    return os.path.join(out_dir, "liboracle_native.so")
+
+
+_libs = {}
+
+
+def _load(path: str = None):
     global _lib
+    if path is not None and path != _LIB_PATH:
+        if path not in _libs:
+            _libs[path] = _bind(ctypes.CDLL(path))
+        return _libs[path]
     if _lib is not None:
         return _lib
     if not os.path.exists(_LIB_PATH):
         build()
-    lib = ctypes.CDLL(_LIB_PATH)
+    _lib = _bind(ctypes.CDLL(_LIB_PATH))
+    return _lib
+
+
+def _bind(lib):
     P = ctypes.c_void_p
     u32, u64, f32, f64, i32 = (ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float,
                                ctypes.c_double, ctypes.c_int)
@@ -92,7 +110,6 @@ def _load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
     return lib
 
 
@@ -111,8 +128,8 @@ def host_feq(ux: float, uy: float, rho: float) -> np.ndarray:
 class RefCPU:
     """Faithful f32 restatement of the reference step (see refcpu.c)."""
 
-    def __init__(self, nx: int, ny: int, omega: float, threads: int = 0):
-        self.lib = _load()
+    def __init__(self, nx: int, ny: int, omega: float, threads: int = 0, lib_path: str = None):
+        self.lib = _load(lib_path)
         if threads:
             self.lib.ref_set_threads(threads)
         self.nx, self.ny = int(nx), int(ny)

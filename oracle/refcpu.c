@@ -46,6 +46,8 @@
 #include <omp.h>
 #endif
 
+/* a target with FMA instructions is allowed only together with -ffp-contract=off
+ * (REFCPU_ALLOW_FMA, set by the Makefile's native build) */
 #if defined(__FP_FAST_FMAF) && !defined(REFCPU_ALLOW_FMA)
 #error "refcpu must be built without FMA contraction"
 #endif
