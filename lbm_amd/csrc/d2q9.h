@@ -202,12 +202,12 @@ struct StoreF16S {
     using T = uint16_t;
     static constexpr int bytes = 2;
     static constexpr bool shifted = false;
-    // load is pinned like store: every kernel shape then performs exactly one
-    // f16 -> f32 conversion per value, never a mixed-precision FMA in its place.
+    // f16 -> f32 is exact, so however the compiler folds the conversion into
+    // its users (a mixed-precision FMA, an SDWA operand) the value is the
+    // same.  (No asm pin here: an inline asm statement is a scheduling
+    // boundary, and pinned loads were waited for one by one.)
     __device__ __forceinline__ static float load(T v, int, const F16Shift &) {
-        float g = (float)__builtin_bit_cast(_Float16, v);
-        asm("" : "+v"(g));
-        return g;
+        return (float)__builtin_bit_cast(_Float16, v);
     }
     // (f - w has the same shift for a population and its opposite)
     __device__ __forceinline__ static float load_b(T v, int k, bool, const F16Shift &sh) { return load(v, k, sh); }
