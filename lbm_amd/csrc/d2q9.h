@@ -211,12 +211,14 @@ struct StoreF16S {
     }
     // (f - w has the same shift for a population and its opposite)
     __device__ __forceinline__ static float load_b(T v, int k, bool, const F16Shift &sh) { return load(v, k, sh); }
-    // The empty asm pins g as an f32 register value, so the conversion is always
-    // a separate f32 -> f16 round to nearest even; otherwise the compiler may fuse
-    // it with the producing FMA (one rounding instead of two) in some kernel
-    // shapes and not others, and kernels would disagree in the last bit.
+    // The format rounds the f32 value to nearest even.  Without a barrier the
+    // compiler folds f16(fma(a, b, c)) into one mixed-precision FMA (a single
+    // rounding instead of two), in some kernel shapes and not others, and
+    // kernels would disagree in the last bit; canonicalize keeps the
+    // conversion separate and costs nothing (an inline asm pin did the same
+    // but was a scheduling boundary).
     __device__ __forceinline__ static T store(float g, int, const F16Shift &) {
-        asm("" : "+v"(g));
+        g = __builtin_elementwise_canonicalize(g);
         return __builtin_bit_cast(uint16_t, (_Float16)g);
     }
     __device__ __forceinline__ static float to_f(T v, int k, const F16Shift &sh) { return load(v, k, sh) + weight(k); }
