@@ -312,7 +312,20 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=5000, help="timed steps of the batched leg")
     ap.add_argument("--variant", type=int, default=None, help="FP32 kernel variant (default: autotune)")
     ap.add_argument("--f16-variant", type=int, default=None, help="FP16-shifted kernel variant (default: autotune)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain copy of 1 GiB each way (y = x * 1.0 over 2^28 f32, three times): the "
+                         "known-size kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE to HBM bytes")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(2 ** 28, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print("copy calibration: 3 x (1 GiB read + 1 GiB written)", flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the D2Q9 step here runs on a single MI355X "
