@@ -38,7 +38,7 @@ struct Args {
 void usage(FILE *to) {
     std::fprintf(to,
                  "usage: lbm_cli [--nx N] [--ny N] [--viscosity NU] [--speed U] [--circle X,Y,R]...\n"
-                 "               [--circles N [--seed S] [--r-min R] [--r-max R]] [--precision f32|f16s]\n"
+                 "               [--circles N [--seed S] [--r-min R] [--r-max R]] [--precision f32|f16s|f16e]\n"
                  "               [--boundary reference|zero-gradient] [--frames F] [--steps-per-frame K]\n"
                  "               [--barrier curl|chaos|welcome] [--variant V] [--dump-f PATH]\n");
 }
@@ -68,8 +68,10 @@ bool parse(int argc, char **argv, Args &a) {
         else if (k == "--r-max") a.r_max = (uint32_t)std::strtoul(val(), nullptr, 10);
         else if (k == "--precision") {
             const std::string v = val();
-            if (v != "f32" && v != "f16s") throw lbm::Error(LBM_EINVAL, "--precision is f32 or f16s");
-            a.precision = v == "f32" ? LBM_PRECISION_F32 : LBM_PRECISION_F16S;
+            if (v == "f32") a.precision = LBM_PRECISION_F32;
+            else if (v == "f16s") a.precision = LBM_PRECISION_F16S;
+            else if (v == "f16e") a.precision = LBM_PRECISION_F16E;
+            else throw lbm::Error(LBM_EINVAL, "--precision is f32, f16s or f16e");
         } else if (k == "--boundary") {
             const std::string v = val();
             if (v != "reference" && v != "zero-gradient")

@@ -160,6 +160,6 @@ def test_cpp_cli_builds_and_parses_arguments_without_a_device():
     r = subprocess.run([CLI, "--help"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "usage: lbm_cli" in r.stdout
     r = subprocess.run([CLI, "--precision", "f8"], capture_output=True, text=True, timeout=60)
-    assert r.returncode == 1 and "--precision is f32 or f16s" in r.stderr
+    assert r.returncode == 1 and "--precision is f32, f16s or f16e" in r.stderr
     r = subprocess.run([CLI, "--bogus"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "unknown option" in r.stderr

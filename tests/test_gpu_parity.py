@@ -673,6 +673,19 @@ def test_cpp_driver_matches_python_and_oracle(tmp_path):
     g.step(60)
     assert same_bits(np.fromfile(dump, np.float32).reshape(9, Y1, X1), g.read_f())
     g.close()
+    # binary16 storage centred on the free-stream equilibrium through the C++ driver
+    r = subprocess.run([cli, "--nx", "400", "--ny", "160", "--viscosity", "0.02", "--speed", "0.1",
+                        "--circle", "80,80,16", "--frames", "1", "--steps-per-frame", "100", "--variant", "8",
+                        "--precision", "f16e", "--dump-f", str(dump)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    g = LBM(float(omega), X1, Y1, precision=2)
+    g.mask = mask
+    g.custom_speed(U1)
+    g.variant = 8
+    g.step(100)
+    assert json.loads(r.stdout.splitlines()[0])["mass"] == g.conservation()[0]
+    assert same_bits(np.fromfile(dump, np.float32).reshape(9, Y1, X1), g.read_f())
+    g.close()
 
 
 def test_cpp_mirror_api(tmp_path):
