@@ -159,6 +159,16 @@ int lbm_sync(lbm_grid *g);
 int lbm_set_variant(lbm_grid *g, int variant);
 int lbm_get_variant(const lbm_grid *g, int *variant);
 int lbm_autotune(lbm_grid *g, uint32_t steps, int *best, float *ms);
+/* lbm_autotune through a JSON file (no reference counterpart): the file maps
+ * (device architecture and CU count, shape class = the grid's width and height
+ * rounded up to powers of two, precision, boundary, number of variants) to the
+ * variant that was fastest.  A key found in the file selects its variant
+ * without timing (*hit = 1); otherwise the call tunes as lbm_autotune and
+ * adds the key (*hit = 0), rewriting the file through a temporary file and a
+ * rename.  A cached two-step variant that cannot get its spare sets counts as
+ * a miss.  LBM_EINVAL when the file cannot be written (the tuned variant is
+ * selected all the same).  best and hit may be NULL. */
+int lbm_autotune_cached(lbm_grid *g, const char *cache_path, uint32_t steps, int *best, int *hit);
 /* lbm_step(g, n) bracketed by two hipEvents recorded on the grid's stream;
  * waits for the second and returns the elapsed device time in *ms.  (No
  * reference counterpart: the reference has no timing, lbm.rs:1166-1172.) */
@@ -263,6 +273,9 @@ int lbm_batch_set_graph(lbm_batch *b, int enable);
 int lbm_batch_set_variant(lbm_batch *b, int variant);
 int lbm_batch_get_variant(const lbm_batch *b, int *variant);
 int lbm_batch_autotune(lbm_batch *b, uint32_t steps, int *best, float *ms);
+/* as lbm_autotune_cached; the shape class is the domain count and the total
+ * cells in MiB-cells, each rounded up to a power of two */
+int lbm_batch_autotune_cached(lbm_batch *b, const char *cache_path, uint32_t steps, int *best, int *hit);
 /* one domain's current state (9 dense nx*ny f32 planes) and renderer (lagged) moments */
 int lbm_batch_read_f(lbm_batch *b, uint32_t domain, float *f9);
 int lbm_batch_read_moments(lbm_batch *b, uint32_t domain, float *jx, float *jy, float *rho);

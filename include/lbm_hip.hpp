@@ -118,6 +118,13 @@ class LBM {
         if (ms_per_variant) *ms_per_variant = ms;
         return best;
     }
+    // autotune through a JSON cache file; *hit tells whether the variant came from the file
+    int autotune_cached(const std::string &cache_path, uint32_t steps, bool *hit = nullptr) {
+        int best = 0, h = 0;
+        call(lbm_autotune_cached(g_, cache_path.c_str(), steps, &best, &h));
+        if (hit) *hit = h != 0;
+        return best;
+    }
     void set_variant(int v) { call(lbm_set_variant(g_, v)); }
     int variant() const {
         int v = 0;

@@ -87,6 +87,8 @@ SIGNATURES = {
     "lbm_set_variant": (_i32, [_P, _i32]),
     "lbm_get_variant": (_i32, [_P, ctypes.POINTER(ctypes.c_int)]),
     "lbm_autotune": (_i32, [_P, _u32, ctypes.POINTER(ctypes.c_int), _fp]),
+    "lbm_autotune_cached": (_i32, [_P, ctypes.c_char_p, _u32, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int)]),
     "lbm_step_count": (_i32, [_P, _u64p]),
     "lbm_get_stream": (_i32, [_P, _PP]),
     "lbm_read_moments": (_i32, [_P, _fp, _fp, _fp, _i32]),
@@ -114,6 +116,8 @@ SIGNATURES = {
     "lbm_batch_set_variant": (_i32, [_P, _i32]),
     "lbm_batch_get_variant": (_i32, [_P, ctypes.POINTER(ctypes.c_int)]),
     "lbm_batch_autotune": (_i32, [_P, _u32, ctypes.POINTER(ctypes.c_int), _fp]),
+    "lbm_batch_autotune_cached": (_i32, [_P, ctypes.c_char_p, _u32, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int)]),
     "lbm_batch_read_f": (_i32, [_P, _u32, _fp]),
     "lbm_batch_read_moments": (_i32, [_P, _u32, _fp, _fp, _fp]),
     "lbm_gen_batch": (_i32, [_u64, _u32, ctypes.POINTER(lbm_domain)]),

@@ -4,6 +4,7 @@ steps one grid; each domain here follows that grid's semantics exactly."""
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -110,6 +111,15 @@ class Batch:
         ms = (ctypes.c_float * N.BATCH_NUM_VARIANTS)()
         self._chk(self._lib.lbm_batch_autotune(self._h, steps, ctypes.byref(best), ms))
         return best.value, list(ms)
+
+    def autotune_cached(self, cache_path: str, steps: int = 10):
+        """autotune through a JSON cache keyed by (device, domain count and
+        total cells rounded to powers of two, precision, boundary).  Returns
+        (best, hit)."""
+        best, hit = ctypes.c_int(), ctypes.c_int()
+        self._chk(self._lib.lbm_batch_autotune_cached(self._h, os.fsencode(cache_path), steps, ctypes.byref(best),
+                                                      ctypes.byref(hit)))
+        return best.value, bool(hit.value)
 
     def read_f(self, d: int) -> np.ndarray:
         nx, ny = self.domains[d][:2]

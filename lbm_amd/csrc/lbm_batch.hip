@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "lbm_common.h"
+#include "tune_cache.h"
 
 namespace {
 
@@ -622,6 +623,39 @@ int lbm_batch_autotune(lbm_batch *b, uint32_t steps, int *best, float *ms) {
         if (ms)
             for (int v = 0; v < LBM_BATCH_NUM_VARIANTS; ++v) ms[v] = t[v] / steps;
     });
+}
+
+int lbm_batch_autotune_cached(lbm_batch *b, const char *cache_path, uint32_t steps, int *best, int *hit) {
+    std::string key;
+    tune_cache::Entry e;
+    bool found = false;
+    int rc = bguard(b, [&] {
+        bcheck(cache_path && *cache_path, "null or empty cache path");
+        bcheck(steps >= 2, "steps must be >= 2");
+        uint64_t cells = 0;
+        for (const DomainDev &d : b->table) cells += (uint64_t)d.g.X * d.g.Y;
+        const uint32_t mcells = (uint32_t)std::min<uint64_t>((cells + (1u << 20) - 1) >> 20, 1u << 30);
+        key = tune_cache::clean_key(device_tag(b->dev) + "/batch/" +
+                                    std::to_string(tune_cache::pow2_ceil((uint32_t)b->table.size())) + "d/" +
+                                    std::to_string(tune_cache::pow2_ceil(mcells)) + "Mcells/" +
+                                    precision_tag(b->precision) +
+                                    (b->boundary == LBM_BOUNDARY_REFERENCE ? "/ref" : "/zg") + "/v" +
+                                    std::to_string(LBM_BATCH_NUM_VARIANTS));
+        found = tune_cache::lookup(cache_path, key, e) && e.variant >= 0 && e.variant < LBM_BATCH_NUM_VARIANTS;
+        if (found) b->variant = e.variant;
+    });
+    if (rc != LBM_OK) return rc;
+    if (hit) *hit = found ? 1 : 0;
+    if (found) {
+        if (best) *best = e.variant;
+        return LBM_OK;
+    }
+    e.ms.assign(LBM_BATCH_NUM_VARIANTS, 0.0f);
+    e.steps = steps;
+    rc = lbm_batch_autotune(b, steps, &e.variant, e.ms.data());
+    if (rc != LBM_OK) return rc;
+    if (best) *best = e.variant;
+    return bguard(b, [&] { bcheck(tune_cache::store(cache_path, key, e), "cannot write the autotune cache file"); });
 }
 
 int lbm_batch_read_f(lbm_batch *b, uint32_t d, float *f9) {

@@ -33,6 +33,7 @@ struct Args {
     int variant = -1;                            // -1: autotune
     std::string barrier;                         // curl, chaos or welcome (lbm.rs:1367-1480)
     std::string dump;
+    std::string tune_cache;                      // JSON autotune cache file
 };
 
 void usage(FILE *to) {
@@ -40,7 +41,7 @@ void usage(FILE *to) {
                  "usage: lbm_cli [--nx N] [--ny N] [--viscosity NU] [--speed U] [--circle X,Y,R]...\n"
                  "               [--circles N [--seed S] [--r-min R] [--r-max R]] [--precision f32|f16s|f16e]\n"
                  "               [--boundary reference|zero-gradient] [--frames F] [--steps-per-frame K]\n"
-                 "               [--barrier curl|chaos|welcome] [--variant V] [--dump-f PATH]\n");
+                 "               [--barrier curl|chaos|welcome] [--variant V] [--tune-cache PATH] [--dump-f PATH]\n");
 }
 
 bool parse(int argc, char **argv, Args &a) {
@@ -81,6 +82,7 @@ bool parse(int argc, char **argv, Args &a) {
         else if (k == "--steps-per-frame") a.steps_per_frame = (uint32_t)std::strtoul(val(), nullptr, 10);
         else if (k == "--variant") a.variant = std::atoi(val());
         else if (k == "--dump-f") a.dump = val();
+        else if (k == "--tune-cache") a.tune_cache = val();
         else if (k == "--barrier") {
             a.barrier = val();
             if (a.barrier != "curl" && a.barrier != "chaos" && a.barrier != "welcome")
@@ -123,7 +125,13 @@ int main(int argc, char **argv) {
         else if (a.barrier == "chaos") lbm::shapes::draw_shape(g, lbm::shapes::chaos_barrier(a.nx, a.ny));
         else if (a.barrier == "welcome") lbm::shapes::draw_shape(g, lbm::shapes::welcome_barrier(a.nx, a.ny));
         g.custom_speed(a.speed);
-        const int variant = a.variant >= 0 ? (g.set_variant(a.variant), a.variant) : g.autotune(60);
+        bool hit = false;
+        const int variant = a.variant >= 0             ? (g.set_variant(a.variant), a.variant)
+                            : !a.tune_cache.empty() ? g.autotune_cached(a.tune_cache, 60, &hit)
+                                                    : g.autotune(60);
+        if (!a.tune_cache.empty() && a.variant < 0)
+            std::fprintf(stderr, "lbm_cli: variant %d %s %s\n", variant, hit ? "read from" : "tuned and stored in",
+                         a.tune_cache.c_str());
         for (uint32_t f = 0; f < a.frames; ++f) {
             g.sync();
             const auto t0 = std::chrono::steady_clock::now();

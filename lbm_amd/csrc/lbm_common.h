@@ -718,6 +718,18 @@ struct ArgError {
     std::string msg;
 };
 
+// "<arch name>/<CU count>cu" of a device: the device part of an autotune
+// cache key (a partitioned device with fewer CUs tunes apart)
+inline std::string device_tag(int dev) {
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, dev));
+    return std::string(p.gcnArchName) + "/" + std::to_string(p.multiProcessorCount) + "cu";
+}
+
+inline const char *precision_tag(int precision) {
+    return precision == LBM_PRECISION_F32 ? "f32" : precision == LBM_PRECISION_F16S ? "f16s" : "f16e";
+}
+
 // Test hook: with LBM_TEST_ALLOC_LIMIT=<bytes> in the environment, a request
 // above that many bytes is passed on as an impossible size, so the runtime
 // fails it the way it fails a real out-of-memory request.

@@ -48,6 +48,7 @@
 #include <vector>
 
 #include "lbm_common.h"
+#include "tune_cache.h"
 
 namespace {
 
@@ -1155,6 +1156,37 @@ int lbm_autotune(lbm_grid *g, uint32_t steps, int *best, float *ms) {
         if (ms)
             for (int v = 0; v < LBM_NUM_VARIANTS; ++v) ms[v] = t[v] / steps;
     });
+}
+
+int lbm_autotune_cached(lbm_grid *g, const char *cache_path, uint32_t steps, int *best, int *hit) {
+    std::string key;
+    tune_cache::Entry e;
+    bool found = false;
+    int rc = guarded(g, [&] {
+        check(cache_path && *cache_path, "null or empty cache path");
+        check(steps > 0, "steps must be > 0");
+        key = tune_cache::clean_key(device_tag(g->dev) + "/grid/" + std::to_string(tune_cache::pow2_ceil(g->X)) +
+                                    "x" + std::to_string(tune_cache::pow2_ceil(g->Y)) + "/" +
+                                    precision_tag(g->precision) +
+                                    (g->boundary == LBM_BOUNDARY_REFERENCE ? "/ref" : "/zg") + "/v" +
+                                    std::to_string(LBM_NUM_VARIANTS));
+        found = tune_cache::lookup(cache_path, key, e) && e.variant >= 0 && e.variant < LBM_NUM_VARIANTS;
+    });
+    if (rc != LBM_OK) return rc;
+    if (hit) *hit = 0;
+    // a cached two-step variant whose spare sets cannot be allocated here is a miss
+    if (found && lbm_set_variant(g, e.variant) == LBM_OK) {
+        if (hit) *hit = 1;
+        if (best) *best = e.variant;
+        g->err.clear();
+        return LBM_OK;
+    }
+    e.ms.assign(LBM_NUM_VARIANTS, 0.0f);
+    e.steps = steps;
+    rc = lbm_autotune(g, steps, &e.variant, e.ms.data());
+    if (rc != LBM_OK) return rc;
+    if (best) *best = e.variant;
+    return guarded(g, [&] { check(tune_cache::store(cache_path, key, e), "cannot write the autotune cache file"); });
 }
 
 int lbm_snapshot(lbm_grid *g) {

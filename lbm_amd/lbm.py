@@ -27,6 +27,7 @@ are host copies.  Errors raise ``LbmError`` (the reference panics instead).
 from __future__ import annotations
 
 import ctypes
+import os
 from enum import IntEnum
 
 import numpy as np
@@ -242,6 +243,15 @@ class LBM:
         ms = (ctypes.c_float * N.NUM_VARIANTS)()
         self._chk(self._lib.lbm_autotune(self._h, steps, ctypes.byref(best), ms))
         return best.value, list(ms)
+
+    def autotune_cached(self, cache_path: str, steps: int = 20):
+        """autotune through a JSON cache keyed by (device, shape class,
+        precision, boundary): a cached key selects its variant without timing.
+        Returns (best, hit)."""
+        best, hit = ctypes.c_int(), ctypes.c_int()
+        self._chk(self._lib.lbm_autotune_cached(self._h, os.fsencode(cache_path), steps, ctypes.byref(best),
+                                                ctypes.byref(hit)))
+        return best.value, bool(hit.value)
 
     def sync(self):
         self._chk(self._lib.lbm_sync(self._h))

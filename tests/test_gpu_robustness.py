@@ -158,3 +158,70 @@ def test_checkpoint_carries_the_inflow_speed():
     net.close()
     a.close()
     c.close()
+
+
+def test_autotune_cache_selects_the_stored_variant(tmp_path):
+    """lbm_autotune_cached / lbm_batch_autotune_cached: a miss tunes (state
+    untouched), stores one key per (device, shape class, precision, boundary)
+    and a grid of the same class then reads its variant without timing; the
+    stored variant, not a fresh tuning, is what a hit selects (an entry edited
+    by hand is obeyed); a two-step variant that cannot get its spare sets is a
+    miss; the C++ driver reads and writes the same file."""
+    import json
+    import subprocess
+    from lbm_amd import LBM, Batch
+    print("case: autotune cache", flush=True)
+    path = tmp_path / "tune.json"
+    mask = circle_mask(400, 160, [(80, 80, 16)])
+    g, ref = _pair(400, 160, mask)
+    g.step(7)
+    ref.step(7)
+    best, hit = g.autotune_cached(str(path), 6)
+    assert not hit and g.variant == best
+    assert same_bits(g.read_f(), ref.read_f()) and g.get_compute_num() == 7
+    data = json.loads(path.read_text())
+    (key, entry), = data.items()
+    assert "/grid/512x256/f32/ref/v10" in key and entry["variant"] == best and len(entry["ms"]) == 10
+    # same class (300 x 200 rounds to 512 x 256): read, not timed
+    h = LBM(1.7, 300, 200)
+    assert h.autotune_cached(str(path), 6) == (best, True)
+    # the file decides: an edited entry is what a hit selects
+    other = 3 if best != 3 else 1
+    data[key]["variant"] = other
+    path.write_text(json.dumps(data))
+    assert h.autotune_cached(str(path), 6) == (other, True) and h.variant == other
+    # a two-step variant whose spare sets cannot be allocated is a miss: the
+    # call tunes afresh, which under the same cap cannot snapshot the state
+    # either, so it fails and changes nothing
+    from lbm_amd._native import LbmError, LBM_ENOMEM as ENOMEM
+    data[key]["variant"] = 8
+    path.write_text(json.dumps(data))
+    with alloc_cap(1 << 16):
+        with pytest.raises(LbmError) as ei:
+            h.autotune_cached(str(path), 6)
+        assert ei.value.code == ENOMEM
+    assert h.variant == other and json.loads(path.read_text())[key]["variant"] == 8
+    assert h.autotune_cached(str(path), 6) == (8, True)
+    # another precision is another key
+    p2 = LBM(1.7, 400, 160, precision=2)
+    assert not p2.autotune_cached(str(path), 6)[1]
+    assert len(json.loads(path.read_text())) == 2
+    for x in (g, ref, h, p2):
+        x.close()
+    # batches
+    doms = [(128, 64, 1.6, 0.05), (256, 128, 1.7, 0.08), (200, 100, 1.8, 0.1)]
+    b = Batch(doms)
+    bb, bhit = b.autotune_cached(str(path), 4)
+    assert not bhit and b.variant == bb
+    b.close()
+    b = Batch(doms)
+    assert b.autotune_cached(str(path), 4) == (bb, True)
+    b.close()
+    assert any("/batch/4d/1Mcells/f32/ref/v6" in k for k in json.loads(path.read_text()))
+    # the C++ driver: tuned and stored, then read
+    cli = os.path.join(os.path.dirname(__file__), "..", "lbm_amd", "lbm_cli")
+    cpath = tmp_path / "cli.json"
+    for expect in ("tuned and stored in", "read from"):
+        r = subprocess.run([cli, "--nx", "400", "--ny", "160", "--frames", "1", "--steps-per-frame", "10",
+                            "--tune-cache", str(cpath)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0 and expect in r.stderr, r.stderr
