@@ -5,14 +5,14 @@
 // reference has no learned component.
 //
 // Convolutions are implicit GEMMs on v_mfma_f32_32x32x16_bf16.  Activations are
-// NHWC; the pixel dimension is the GEMM's M, output channels its N and
-// (tap, input channel) its K.  For one wave:
-//   A (32 pixels x 16 k):  lane l holds pixel l % 32, k = 8 (l / 32) + 0..7
-//                          -> 16 contiguous bytes of one pixel's channels;
-//   B (16 k x 32 couts):   lane l holds cout l % 32, k = 8 (l / 32) + 0..7
-//                          -> 16 contiguous bytes of W[tap][cout][cin];
-//   D (32 x 32 f32):       lane l, value r: pixel 8 (r / 4) + 4 (l / 32) + r % 4,
-//                          cout l % 32.
+// NHWC; (tap, input channel) is the GEMM's K.  Both operands of one product
+// have the same lane layout -- lane l holds row l % 32 and k = 8 (l / 32) +
+// 0..7, i.e. 16 contiguous bytes of one pixel's channels or of one row of
+// W[tap][cout][cin] -- so which of them is "A" only decides the layout of the
+// result.  The 1x1 kernel puts pixels in A: D (32 x 32 f32), lane l, value r
+// holds pixel 8 (r / 4) + 4 (l / 32) + r % 4, cout l % 32.  The 3x3 kernels put
+// the weights in A: lane l holds pixel l % 32 and couts 8 (r / 4) + 4 (l / 32)
+// + r % 4, four runs of four consecutive channels that are stored as vectors.
 // The input gradient is the same forward convolution applied to the output
 // gradient with the weights flipped in space and transposed in channels; the
 // weight gradient reduces over pixels (K) with both operands gathered per lane.
@@ -133,6 +133,158 @@ __global__ __launch_bounds__(256) void k_conv1x1(const uint16_t *__restrict__ in
     }
 }
 
+// Epilogue of the 3x3 kernels.  The products are issued with the weights as
+// the A operand and the pixels as B, so a wave's 32 x 32 result is
+// [cout][pixel]: lane l holds pixel l % 32 and output channels
+// 8 (r / 4) + 4 (l / 32) + r % 4, r = 0..15 -- four runs of four consecutive
+// channels, stored as four 8-byte (bf16) or 16-byte (f32) pieces.  A wave
+// covers MI rows (from y_first) x 32 columns (from x0) x NI tiles of 32
+// output channels (from co_first).  With a K split (gridDim.z > 1) the raw
+// sums go to part[z][p][co]; otherwise bias, ReLU and the store of channels
+// co < co_store.
+template <int MI, int NI>
+__device__ __forceinline__ void conv_store_tile(const f32x16 (&acc)[MI][NI], const float *__restrict__ bias,
+                                                void *__restrict__ out, float *__restrict__ part, int N, int H, int W,
+                                                int Cout, int co_store, int relu, int out_f32, long long n, int x0,
+                                                int y_first, int co_first, int half, int l32) {
+    const long long P = (long long)N * H * W;
+    const bool split = gridDim.z > 1;
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+        const int y = y_first + i;
+        if (y >= H) continue;
+        const long long p = (n * H + y) * W + x0 + l32;
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int co = co_first + 32 * j + 8 * g + 4 * half;
+                float v[4] = {acc[i][j][4 * g], acc[i][j][4 * g + 1], acc[i][j][4 * g + 2], acc[i][j][4 * g + 3]};
+                if (split) {
+                    *(float4 *)&part[((long long)blockIdx.z * P + p) * Cout + co] = make_float4(v[0], v[1], v[2], v[3]);
+                    continue;
+                }
+                if (bias) {
+                    const float4 bv = *(const float4 *)&bias[co];
+                    v[0] += bv.x;
+                    v[1] += bv.y;
+                    v[2] += bv.z;
+                    v[3] += bv.w;
+                }
+                if (relu)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) v[e] = fmaxf(v[e], 0.0f);
+                if (co + 4 <= co_store) {
+                    if (out_f32) {
+                        *(float4 *)&((float *)out)[p * Cout + co] = make_float4(v[0], v[1], v[2], v[3]);
+                    } else {
+                        const uint32_t lo = f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                                       hi = f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                        *(uint2 *)&((uint16_t *)out)[p * Cout + co] = make_uint2(lo, hi);
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (co + e < co_store) {
+                            if (out_f32) ((float *)out)[p * Cout + co + e] = v[e];
+                            else ((uint16_t *)out)[p * Cout + co + e] = f2bf(v[e]);
+                        }
+                }
+            }
+        }
+    }
+}
+
+// 2 x 2 average pool of a block's bf16 outputs into pout (H / 2 x W / 2 x
+// Cout), with k_pool2's arithmetic on the stored values:
+// bf16(0.25 ((v(y, x) + v(y, x + 1)) + (v(y + 1, x) + v(y + 1, x + 1)))).
+// Horizontal neighbours are adjacent lanes; a wave holding two rows (MI == 2)
+// has both vertical neighbours, otherwise (one row per wave, four waves down
+// the block) odd waves pass their row to the wave above through `scratch`
+// (4 KB per 32-channel tile; free after the main loop's last barrier).
+// Every thread of the block must call it.
+template <int MI, int NI, int WN>
+__device__ __forceinline__ void conv_pool_tile(const f32x16 (&acc)[MI][NI], const float *__restrict__ bias,
+                                               uint16_t *__restrict__ pout, uint16_t *scratch, int H, int W, int Cout,
+                                               int relu, long long n, int x0, int y_first, int wm, int co_first,
+                                               int lane) {
+    static_assert(MI == 2 || (MI == 1 && WN == 1), "pooling needs row pairs inside a wave or a block column");
+    const int half = lane >> 5, l32 = lane & 31;
+    const int Ho = H >> 1, Wo = W >> 1;
+    // the stored (bf16-rounded) output of row i, tile j, channels 8 g + 4 half + e
+    auto vals = [&](int i, int j, int g, float (&v)[4]) {
+        const int co = co_first + 32 * j + 8 * g + 4 * half;
+        const float4 bv = bias ? *(const float4 *)&bias[co] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        v[0] = acc[i][j][4 * g] + bv.x;
+        v[1] = acc[i][j][4 * g + 1] + bv.y;
+        v[2] = acc[i][j][4 * g + 2] + bv.z;
+        v[3] = acc[i][j][4 * g + 3] + bv.w;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = bf2f(f2bf(relu ? fmaxf(v[e], 0.0f) : v[e]));
+    };
+    auto emit = [&](int y, int j, int g, const float (&top)[4], const float (&bot)[4]) {
+        float o[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float t1 = __shfl_xor(top[e], 1), b1 = __shfl_xor(bot[e], 1);
+            o[e] = 0.25f * ((top[e] + t1) + (bot[e] + b1));
+        }
+        if ((l32 & 1) == 0 && y < H) {
+            const int co = co_first + 32 * j + 8 * g + 4 * half;
+            const long long q = (n * Ho + (y >> 1)) * Wo + ((x0 + l32) >> 1);
+            const uint32_t lo = f2bf(o[0]) | ((uint32_t)f2bf(o[1]) << 16), hi = f2bf(o[2]) | ((uint32_t)f2bf(o[3]) << 16);
+            *(uint2 *)&pout[q * Cout + co] = make_uint2(lo, hi);
+        }
+    };
+    if constexpr (MI == 2) {
+#pragma unroll
+        for (int j = 0; j < NI; ++j)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                float top[4], bot[4];
+                vals(0, j, g, top);
+                vals(1, j, g, bot);
+                emit(y_first, j, g, top, bot);
+            }
+    } else {
+        u16x8 *slot = (u16x8 *)scratch + (size_t)(wm >> 1) * NI * 128;
+        if (wm & 1) {
+#pragma unroll
+            for (int j = 0; j < NI; ++j)
+#pragma unroll
+                for (int h2 = 0; h2 < 2; ++h2) {
+                    float a[4], b[4];
+                    vals(0, j, 2 * h2, a);
+                    vals(0, j, 2 * h2 + 1, b);
+                    u16x8 pk;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        pk[e] = f2bf(a[e]);
+                        pk[4 + e] = f2bf(b[e]);
+                    }
+                    slot[(j * 2 + h2) * 64 + lane] = pk;
+                }
+        }
+        __syncthreads();
+        if (!(wm & 1)) {
+#pragma unroll
+            for (int j = 0; j < NI; ++j)
+#pragma unroll
+                for (int h2 = 0; h2 < 2; ++h2) {
+                    const u16x8 pk = slot[(j * 2 + h2) * 64 + lane];
+#pragma unroll
+                    for (int gg = 0; gg < 2; ++gg) {
+                        float top[4], bot[4];
+                        vals(0, j, 2 * h2 + gg, top);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) bot[e] = bf2f(pk[4 * gg + e]);
+                        emit(y_first, j, 2 * h2 + gg, top, bot);
+                    }
+                }
+        }
+    }
+}
+
 // LDS-staged 3x3 convolution.  A block computes TH = (4 / WN) * MI rows x 32
 // columns of output pixels of one image for 32 * WN * NI output channels; its
 // four waves form a (4 / WN) x WN grid, each computing MI rows x NI tiles of 32
@@ -149,7 +301,8 @@ __global__ __launch_bounds__(256) void k_conv1x1(const uint16_t *__restrict__ in
 template <int MI, int NI, int WN>
 __global__ __launch_bounds__(256) void k_conv3_lds(ConvIn src, const uint16_t *__restrict__ w,
                                                    const float *__restrict__ bias, void *__restrict__ out,
-                                                   float *__restrict__ part, int N, int H, int W, int Cout,
+                                                   float *__restrict__ part, uint16_t *__restrict__ pout, int N, int H,
+                                                   int W, int Cout,
                                                    int co_store, int relu, int out_f32) {
     constexpr int WM = 4 / WN, TH = WM * MI, HR = TH + 2, HC = 34, PS = 40;
     constexpr int NPIECE = HR * HC * 4;  // 16-byte pieces of one chunk's halo
@@ -228,7 +381,7 @@ __global__ __launch_bounds__(256) void k_conv3_lds(ConvIn src, const uint16_t *_
 #pragma unroll
                     for (int j = 0; j < NI; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            __builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, bc[j][s2]), acc[i][j], 0, 0, 0);
+                            __builtin_bit_cast(bf16x8, bc[j][s2]), __builtin_bit_cast(bf16x8, a), acc[i][j], 0, 0, 0);
                 }
             }
             if (tap < 8) {
@@ -250,33 +403,136 @@ __global__ __launch_bounds__(256) void k_conv3_lds(ConvIn src, const uint16_t *_
         if (ck + 1 < ck1) swrite(buf ^ 1);
         __syncthreads();
     }
-    const long long P = (long long)N * H * W;
-    const bool split = gridDim.z > 1;
+    conv_store_tile<MI, NI>(acc, bias, out, part, N, H, W, Cout, co_store, relu, out_f32, n, x0, y0 + wm * MI, co0,
+                            half, l32);
+    if constexpr (MI == 2 || (MI == 1 && WN == 1))
+        if (pout) conv_pool_tile<MI, NI, WN>(acc, bias, pout, &lds[0][0], H, W, Cout, relu, n, x0, y0 + wm * MI, wm, co0, lane);
+}
+
+// The same convolution with the weights staged in LDS too.  Per chunk the
+// block loads the B operands of all nine taps for its 32 * WN * NI output
+// channels once, with 16-byte loads along input channels (64 contiguous bytes
+// per (tap, cout)), into an 80-byte-stride table beside the halo; the waves
+// read them from there.  This replaces every wave's own per-tap loads of 32
+// scattered 32-byte segments, which (not the products, nor the halo) bound the
+// kernel above once a layer has several input chunks.  Both tables are double
+// buffered and filled from registers loaded during the previous chunk's
+// products.  Block shapes with 32 * WN * NI <= 64 output channels only (LDS).
+template <int MI, int NI, int WN>
+__global__ __launch_bounds__(256) void k_conv3_lw(ConvIn src, const uint16_t *__restrict__ w,
+                                                  const float *__restrict__ bias, void *__restrict__ out,
+                                                  float *__restrict__ part, uint16_t *__restrict__ pout, int N, int H,
+                                                   int W, int Cout,
+                                                  int co_store, int relu, int out_f32) {
+    constexpr int WM = 4 / WN, TH = WM * MI, HR = TH + 2, HC = 34, PS = 40;
+    constexpr int CB = 32 * NI * WN;     // output channels of the block
+    static_assert(CB <= 64, "weight table too large for LDS");
+    constexpr int NPIECE = HR * HC * 4;  // 16-byte pieces of one chunk's halo
+    constexpr int PER_T = (NPIECE + 255) / 256;
+    constexpr int WPIECE = 9 * CB * 4;   // 16-byte pieces of one chunk's weights
+    constexpr int WPER_T = (WPIECE + 255) / 256;
+    constexpr int HALO = HR * HC * PS, WTAB = 9 * CB * PS;
+    __shared__ __align__(16) uint16_t lds[2][HALO + WTAB];
+    const int Cin = src.ca + src.cb;
+    const int tiles_x = W >> 5, tiles_y = (H + TH - 1) / TH;
+    int bid = blockIdx.x;
+    const int tx = bid % tiles_x;
+    bid /= tiles_x;
+    const int ty = bid % tiles_y;
+    const long long n = bid / tiles_y;
+    const int x0 = tx * 32, y0 = ty * TH;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int cb0 = blockIdx.y * CB;
+    const int up = src.a_up ? 1 : 0, Ha = H >> up, Wa = W >> up;
+    const int nck = Cin >> 5;
+    const int ck0 = (int)(blockIdx.z * nck / gridDim.z), ck1 = (int)((blockIdx.z + 1) * nck / gridDim.z);
+
+    u16x8 stage[PER_T], wstage[WPER_T];
+    auto gload = [&](int ck) {
+        const int c0 = ck << 5;
 #pragma unroll
-    for (int i = 0; i < MI; ++i) {
-        const int y = y0 + wm * MI + i;
-        if (y >= H) continue;
-#pragma unroll
-        for (int j = 0; j < NI; ++j) {
-            const int co = co0 + 32 * j + l32;
-            const float bv = (!split && bias) ? bias[co] : 0.0f;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int x = x0 + 8 * (r / 4) + 4 * half + (r % 4);
-                const long long p = (n * H + y) * W + x;
-                if (split) {
-                    part[((long long)blockIdx.z * P + p) * Cout + co] = acc[i][j][r];
-                } else {
-                    float v = acc[i][j][r] + bv;
-                    if (relu) v = fmaxf(v, 0.0f);
-                    if (co < co_store) {
-                        if (out_f32) ((float *)out)[p * Cout + co] = v;
-                        else ((uint16_t *)out)[p * Cout + co] = f2bf(v);
-                    }
+        for (int t = 0; t < PER_T; ++t) {
+            const int q = threadIdx.x + 256 * t;
+            stage[t] = u16x8{};
+            if (q < NPIECE) {
+                const int pi = q >> 2, c = c0 + ((q & 3) << 3);
+                const int y = y0 - 1 + pi / HC, x = x0 - 1 + pi % HC;
+                if (y >= 0 && y < H && x >= 0 && x < W) {
+                    const uint16_t *ptr = c < src.ca
+                        ? src.a + ((n * Ha + (y >> up)) * Wa + (x >> up)) * src.ca + c
+                        : src.b + ((n * H + y) * W + x) * src.cb + (c - src.ca);
+                    stage[t] = *(const u16x8 *)ptr;
                 }
             }
         }
+        // weights: piece q = (tap * CB + cout) * 4 + 16-byte piece of the chunk
+#pragma unroll
+        for (int t = 0; t < WPER_T; ++t) {
+            const int q = threadIdx.x + 256 * t;
+            if (q < WPIECE) {
+                const int row = q >> 2, tap = row / CB, cl = row % CB;
+                wstage[t] = *(const u16x8 *)(w + ((size_t)tap * Cout + cb0 + cl) * Cin + c0 + ((q & 3) << 3));
+            }
+        }
+    };
+    auto swrite = [&](int buf) {
+#pragma unroll
+        for (int t = 0; t < PER_T; ++t) {
+            const int q = threadIdx.x + 256 * t;
+            if (q < NPIECE) *(u16x8 *)&lds[buf][(q >> 2) * PS + ((q & 3) << 3)] = stage[t];
+        }
+#pragma unroll
+        for (int t = 0; t < WPER_T; ++t) {
+            const int q = threadIdx.x + 256 * t;
+            if (q < WPIECE) *(u16x8 *)&lds[buf][HALO + (q >> 2) * PS + ((q & 3) << 3)] = wstage[t];
+        }
+    };
+    f32x16 acc[MI][NI];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NI; ++j) acc[i][j] = f32x16{};
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2) {
+                u16x8 b[NI];
+#pragma unroll
+                for (int j = 0; j < NI; ++j)
+                    b[j] = *(const u16x8 *)&lds[buf][HALO + (tap * CB + (wn * NI + j) * 32 + l32) * PS + 16 * s2 +
+                                                     8 * half];
+#pragma unroll
+                for (int i = 0; i < MI; ++i) {
+                    const int r = wm * MI + i + dy;
+                    const u16x8 a = *(const u16x8 *)&lds[buf][(r * HC + l32 + dx) * PS + 16 * s2 + 8 * half];
+#pragma unroll
+                    for (int j = 0; j < NI; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            __builtin_bit_cast(bf16x8, b[j]), __builtin_bit_cast(bf16x8, a), acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+    };
+    gload(ck0);
+    swrite(0);
+    __syncthreads();
+    for (int ck = ck0; ck < ck1; ++ck) {
+        const int buf = (ck - ck0) & 1;
+        if (ck + 1 < ck1) gload(ck + 1);
+        compute(buf);
+        if (ck + 1 < ck1) swrite(buf ^ 1);
+        __syncthreads();
     }
+    conv_store_tile<MI, NI>(acc, bias, out, part, N, H, W, Cout, co_store, relu, out_f32, n, x0, y0 + wm * MI,
+                            cb0 + wn * NI * 32, half, l32);
+    if constexpr (MI == 2 || (MI == 1 && WN == 1))
+        if (pout)
+            conv_pool_tile<MI, NI, WN>(acc, bias, pout, &lds[0][0], H, W, Cout, relu, n, x0, y0 + wm * MI, wm,
+                                       cb0 + wn * NI * 32, lane);
 }
 
 // out = act(bias + sum of the split partials), four channels per thread
@@ -572,11 +828,11 @@ constexpr int kTiles3[13][3] = {{1, 1, 1}, {2, 1, 1}, {4, 1, 1}, {1, 2, 1}, {2, 
 // Register tile / block shape and K split of a convolution launch.  tile: index
 // into kTiles3 plus one (0: choose); split: input-chunk splits (0: choose).
 struct ConvTile {
-    int tile = 0, split = 0;
+    int tile = 0, split = 0, kind = -1;  // kind: 0 weights per wave from global, 1 weights in LDS, -1 choose
 };
 
 struct ConvPlan {
-    int tile, split;
+    int tile, split, kind;
 };
 
 long long conv3_blocks(int t, int N, int H, int W, int Cout) {
@@ -591,8 +847,19 @@ long long conv3_blocks(int t, int N, int H, int W, int Cout) {
 // fewer than 256 blocks (one per CU) falls back to a narrower one, then to a
 // split of the input chunks.
 ConvPlan conv3_plan(int N, int H, int W, int Cin, int Cout) {
+    // Weights staged in LDS (k_conv3_lw) on one-row-per-wave, 32-channel
+    // blocks: fastest for every layer shape of the network except 32 -> 32
+    // channels (one input chunk, where the halo-only kernel keeps twice the
+    // blocks resident); K split while the blocks cover fewer than half the
+    // CUs (at 128..255 blocks the split's extra pass costs more than it gains).
+    if (!(Cin == 32 && Cout == 32)) {
+        int split = 1;
+        const int nck = Cin / 32;
+        while (split < 4 && 2 * split <= nck && conv3_blocks(0, N, H, W, Cout) * split < 128) split *= 2;
+        return {0, split, 1};
+    }
     // preference lists, indices into kTiles3
-    static const int p32a[] = {0, 1, 2};           // Cout 32, Cin 32
+    static const int p32a[] = {1, 0, 2};           // Cout 32, Cin 32
     static const int p32b[] = {1, 2, 0};           // Cout 32, Cin > 32
     static const int p64a[] = {5, 4, 0};           // Cout 64, Cin 32
     static const int p64b[] = {4, 6, 1, 0};        // Cout 64, Cin > 32
@@ -626,7 +893,7 @@ ConvPlan conv3_plan(int N, int H, int W, int Cin, int Cout) {
     int split = 1;
     const int nck = Cin / 32;
     while (split < 4 && 2 * split <= nck && conv3_blocks(best, N, H, W, Cout) * split < 256) split *= 2;
-    return {best, split};
+    return {best, split, 0};
 }
 
 // f32 elements of split-K partial sums a 3x3 convolution needs (0: no split)
@@ -641,14 +908,14 @@ size_t conv_split_elems(int ks, long long P, int N, int H, int W, int Cin, int C
 // the K split is taken).
 void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const float *bias, void *out, int N, int H,
               int W, int Cout, bool relu, bool out_f32, int co_store, float *ws, size_t ws_elems,
-              ConvTile force = ConvTile{}) {
+              ConvTile force = ConvTile{}, uint16_t *pool_out = nullptr) {
     const long long P = (long long)N * H * W;
     const int Cin = in.ca + in.cb;
     ucheck(W % 32 == 0 && in.ca % 32 == 0 && in.cb % 32 == 0 && Cin > 0 && Cout % 32 == 0,
            "convolution shape: W, channel counts must be multiples of 32");
     ucheck(!in.a_up || H % 2 == 0, "upsampled input needs an even height");
     if (ks == 1) {
-        ucheck(!in.a_up && in.cb == 0, "1x1 convolutions take one plain input");
+        ucheck(!in.a_up && in.cb == 0 && !pool_out, "1x1 convolutions take one plain input, no pooled output");
         int mi = 1, ni = 1;
         if (P % 64 == 0 && P / 64 * (Cout / 32) >= 4096) mi = 2;
         else if (Cout % 64 == 0 && P / 32 * (Cout / 64) >= 4096) ni = 2;
@@ -671,19 +938,37 @@ void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const 
         return;
     }
     ConvPlan pl = conv3_plan(N, H, W, Cin, Cout);
-    if (force.tile) pl.tile = force.tile - 1;
+    if (force.tile) {  // a forced block shape runs on the halo-only kernel unless the kind is forced too
+        pl.tile = force.tile - 1;
+        pl.kind = 0;
+    }
+    if (force.kind >= 0) pl.kind = force.kind;
     if (force.split) pl.split = force.split;
     const int t = pl.tile;
     ucheck(t >= 0 && t < 13 && Cout % (32 * kTiles3[t][1] * kTiles3[t][2]) == 0, "unsupported block shape");
     ucheck(pl.split == 1 || pl.split == 2 || pl.split == 4, "unsupported split");
+    while (pl.split > Cin / 32) pl.split /= 2;  // every split gets at least one input chunk
     const bool split = pl.split > 1;
     ucheck(!split || (ws && ws_elems >= (size_t)pl.split * P * Cout), "convolution split workspace too small");
     const dim3 grid((unsigned)conv3_blocks(t, N, H, W, Cout) / (Cout / (32 * kTiles3[t][1] * kTiles3[t][2])),
                     (unsigned)(Cout / (32 * kTiles3[t][1] * kTiles3[t][2])), (unsigned)pl.split);
     const int r = relu ? 1 : 0, f = out_f32 ? 1 : 0;
-#define L3(MI, NI, WN)                                                                                          \
-    hipLaunchKernelGGL((k_conv3_lds<MI, NI, WN>), grid, dim3(256), 0, s, in, w, bias, out, ws, N, H, W, Cout, \
-                       co_store, r, f)
+    const int kind = pl.kind;
+    ucheck(kind == 0 || 32 * kTiles3[t][1] * kTiles3[t][2] <= 64, "block shape too wide for LDS weights");
+    // the 2 x 2 average pool of the output rides in the epilogue when the block
+    // holds row pairs and the output is final (no K split); otherwise k_pool2
+    ucheck(!pool_out || (!out_f32 && co_store == Cout && H % 2 == 0), "pooled output needs a full bf16 output");
+    const bool fuse_pool = pool_out && !split && (kTiles3[t][0] == 2 || (kTiles3[t][0] == 1 && kTiles3[t][2] == 1));
+    uint16_t *pout = fuse_pool ? pool_out : nullptr;
+#define L3(MI, NI, WN)                                                                                             \
+    do {                                                                                                           \
+        if (kind == 1 && (MI) * 0 + 32 * (NI) * (WN) <= 64)                                                        \
+            hipLaunchKernelGGL((k_conv3_lw<MI, NI, (32 * NI * WN <= 64 ? WN : 1)>), grid, dim3(256), 0, s, in, w,   \
+                               bias, out, ws, pout, N, H, W, Cout, co_store, r, f);                                \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_conv3_lds<MI, NI, WN>), grid, dim3(256), 0, s, in, w, bias, out, ws, pout, N, H, \
+                               W, Cout, co_store, r, f);                                                           \
+    } while (0)
     switch (t) {
         case 0: L3(1, 1, 1); break;
         case 1: L3(2, 1, 1); break;
@@ -710,6 +995,11 @@ void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const 
             hipLaunchKernelGGL((k_conv_finish<true, false>), fg, dim3(256), 0, s, ws, bias, out, n, Cout, pl.split);
         else
             hipLaunchKernelGGL((k_conv_finish<false, false>), fg, dim3(256), 0, s, ws, bias, out, n, Cout, pl.split);
+        HIPCHK(hipGetLastError());
+    }
+    if (pool_out && !fuse_pool) {
+        hipLaunchKernelGGL(k_pool2, dim3(nblk((size_t)P * Cout / 32)), dim3(256), 0, s, (const uint16_t *)out,
+                           pool_out, N, H, W, Cout);
         HIPCHK(hipGetLastError());
     }
 }
@@ -948,20 +1238,21 @@ struct lbm_unet {
                  ws_elems);
     }
     void fwd_conv(int i, const Act &in, const Act &out) { conv(i, one_input(in.a, in.C), in.H, in.W, out.a); }
-    void pool(const Act &in, const Act &out) {
-        hipLaunchKernelGGL(k_pool2, dim3(nblk(out.n() / 8)), dim3(256), 0, stream, in.a, out.a, B, in.H, in.W, in.C);
+    // conv i into out, and out's 2 x 2 average pool into pooled
+    void fwd_conv_pool(int i, const Act &in, const Act &out, const Act &pooled) {
+        const Conv &c = convs[i];
+        ucheck(in.C == c.cin && pooled.C == c.cout && pooled.H * 2 == in.H && pooled.W * 2 == in.W, "pooled layer");
+        conv_fwd(stream, c.ks, one_input(in.a, in.C), wb + c.wboff, w32 + c.boff, out.a, B, in.H, in.W, c.cout, c.relu,
+                 false, c.cout, ws, ws_elems, ConvTile{}, pooled.a);
     }
 
     void forward() {
         fwd_conv(0, x, e1a);
-        fwd_conv(1, e1a, e1);
-        pool(e1, p1);
+        fwd_conv_pool(1, e1a, e1, p1);
         fwd_conv(2, p1, e2a);
-        fwd_conv(3, e2a, e2);
-        pool(e2, p2);
+        fwd_conv_pool(3, e2a, e2, p2);
         fwd_conv(4, p2, e3a);
-        fwd_conv(5, e3a, e3);
-        pool(e3, p3);
+        fwd_conv_pool(5, e3a, e3, p3);
         fwd_conv(6, p3, e4a);
         fwd_conv(7, e4a, e4);
         fwd_conv(8, e4, d3u);
@@ -1263,9 +1554,11 @@ int lbm_unet_time_forward(lbm_unet *n, int reps, float *ms) {
 }
 
 // Test entry point for the convolution primitives on host arrays (f32 in,
-// rounded to bf16 on the device).  mode 0: out = conv(in, w) + bias (no ReLU);
-// mode 1: out = input gradient for output gradient `in`; mode 2: out = weight
-// gradient dw[tap][co][ci] for input `in` and output gradient `g`.
+// rounded to bf16 on the device).  mode 0: out = conv(in, w) + bias (no ReLU;
+// with bit 12 also its pooled bf16 output); mode 1: out = input gradient for
+// output gradient `in`; mode 2: out = weight gradient dw[tap][co][ci] for input
+// `in` and output gradient `g`; mode 3: timing.  Bits 4..11 force the block
+// shape, split and kernel.
 int lbm_conv3x3_test(int mode, int N, int H, int W, int Cin, int Cout, const float *in, const float *g,
                      const float *w, const float *bias, float *out, int device) {
     return uguard([&] {
@@ -1273,7 +1566,10 @@ int lbm_conv3x3_test(int mode, int N, int H, int W, int Cin, int Cout, const flo
         ConvTile force;
         force.tile = (mode >> 4) & 15;
         force.split = (mode >> 8) & 7;
-        ucheck(force.tile <= 13 && (mode >> 11) == 0, "mode");
+        force.kind = (mode >> 11) & 1 ? 1 : -1;
+        const bool pooled = (mode >> 12) & 1;
+        ucheck(force.tile <= 13 && (mode >> 13) == 0, "mode");
+        ucheck(!pooled || ((mode & 3) == 0 && H % 2 == 0), "mode: pooled output needs the forward and an even height");
         ucheck(force.split == 0 || force.split == 1 || force.split == 2 || force.split == 4, "mode: split");
         mode = op;
         ucheck(W % 32 == 0 && Cin % 32 == 0 && Cout % 32 == 0, "need W, Cin, Cout multiples of 32");
@@ -1324,8 +1620,20 @@ int lbm_conv3x3_test(int mode, int N, int H, int W, int Cin, int Cout, const flo
                     db = (float *)alloc(Cout * sizeof(float));
                     HIPCHK(hipMemcpy(db, bias, Cout * sizeof(float), hipMemcpyHostToDevice));
                 }
-                conv_fwd(0, 3, one_input(din, Cin), dw, db, dout, N, H, W, Cout, false, true, Cout, ws, wsn, force);
-                HIPCHK(hipMemcpy(out, dout, P * Cout * sizeof(float), hipMemcpyDeviceToHost));
+                if (pooled) {  // bf16 output and its 2 x 2 average pool, both returned as f32
+                    uint16_t *o16 = (uint16_t *)alloc(P * Cout * 2), *p16 = (uint16_t *)alloc(P / 4 * Cout * 2);
+                    conv_fwd(0, 3, one_input(din, Cin), dw, db, o16, N, H, W, Cout, false, false, Cout, ws, wsn, force,
+                             p16);
+                    hipLaunchKernelGGL(k_bf16_to_f32, dim3(nblk(P * Cout)), dim3(256), 0, 0, o16, dout, P * Cout);
+                    HIPCHK(hipMemcpy(out, dout, P * Cout * sizeof(float), hipMemcpyDeviceToHost));
+                    hipLaunchKernelGGL(k_bf16_to_f32, dim3(nblk(P / 4 * Cout)), dim3(256), 0, 0, p16, dout,
+                                       P / 4 * Cout);
+                    HIPCHK(hipMemcpy(out + P * Cout, dout, P / 4 * Cout * sizeof(float), hipMemcpyDeviceToHost));
+                } else {
+                    conv_fwd(0, 3, one_input(din, Cin), dw, db, dout, N, H, W, Cout, false, true, Cout, ws, wsn,
+                             force);
+                    HIPCHK(hipMemcpy(out, dout, P * Cout * sizeof(float), hipMemcpyDeviceToHost));
+                }
             } else if (mode == 1) {
                 uint16_t *dwt = (uint16_t *)alloc(nw * 2);
                 hipLaunchKernelGGL(k_flip_transpose, dim3(nblk(nw)), dim3(256), 0, 0, dw, dwt, Cin, Cout, 9);

@@ -59,12 +59,12 @@ def test_bf16_rounding_helper():
     assert abs(r[3] + 3.140625) < 1e-12
 
 
-def _run(mode, N, H, W, Ci, Co, x, g, w, b):
+def _run(mode, N, H, W, Ci, Co, x, g, w, b, size=None):
     from lbm_amd import _native as Nn
     lib = Nn.load()
     fp = lambda a: None if a is None else np.ascontiguousarray(a, np.float32).ctypes.data_as(  # noqa: E731
         ctypes.POINTER(ctypes.c_float))
-    size = {0: N * H * W * Co, 1: N * H * W * Ci, 2: 9 * Co * Ci}[mode & 3]
+    size = size or {0: N * H * W * Co, 1: N * H * W * Ci, 2: 9 * Co * Ci}[mode & 3]
     out = np.zeros(size, np.float32)
     keep = [np.ascontiguousarray(a, np.float32) if a is not None else None for a in (x, g, w, b)]
     rc = lib.lbm_conv3x3_test(mode, N, H, W, Ci, Co, *[fp(a) for a in keep],
@@ -91,17 +91,36 @@ def test_mfma_conv_matches_oracle(N, H, W, Ci, Co):
     ref_f = cr.conv3x3(x, w, b)
     ref_d = cr.conv3x3_dgrad(g, w)
     scale = np.sqrt(9 * Ci) * 0.1
-    for tile in range(len(SHAPES) + 1):
+    # kind 1: the kernel that stages the weights in LDS (block shapes with <= 64 output channels)
+    cases = [(tile, 0) for tile in range(len(SHAPES) + 1)]
+    cases += [(tile, 1) for tile in range(1, len(SHAPES) + 1) if 32 * SHAPES[tile - 1][1] * SHAPES[tile - 1][2] <= 64]
+    for tile, kind in cases:
         nw = 32 * SHAPES[tile - 1][1] * SHAPES[tile - 1][2] if tile else 32
         for split in (0, 1, 2, 4):
+            mode = tile << 4 | split << 8 | kind << 11
             # forward: bf16 inputs are exact, accumulation in f32
             if Co % nw == 0:
-                got = _run(0 | tile << 4 | split << 8, N, H, W, Ci, Co, x, None, w, b).reshape(N, H, W, Co)
-                assert np.abs(got - ref_f).max() < 1e-5 * scale * 10, (tile, split)
+                got = _run(0 | mode, N, H, W, Ci, Co, x, None, w, b).reshape(N, H, W, Co)
+                assert np.abs(got - ref_f).max() < 1e-5 * scale * 10, (tile, split, kind)
             # input gradient: the same kernel with Cin and Cout exchanged
             if Ci % nw == 0:
-                got = _run(1 | tile << 4 | split << 8, N, H, W, Ci, Co, g, None, w, None).reshape(N, H, W, Ci)
-                assert np.abs(got - ref_d).max() < 1e-4 * np.sqrt(9 * Co), (tile, split)
+                got = _run(1 | mode, N, H, W, Ci, Co, g, None, w, None).reshape(N, H, W, Ci)
+                assert np.abs(got - ref_d).max() < 1e-4 * np.sqrt(9 * Co), (tile, split, kind)
+    # the 2 x 2 average pool in the convolution's epilogue equals k_pool2's arithmetic on the
+    # stored bf16 output, bit for bit (row pairs in a wave, or in two waves through LDS)
+    if H % 2 == 0:
+        # every block shape: those without row pairs (and K splits) take the separate pool pass
+        pool_cases = [(t, k) for t, k in cases if not t or Co % (32 * SHAPES[t - 1][1] * SHAPES[t - 1][2]) == 0]
+        for tile, kind in pool_cases:
+            for split in (0, 2):
+                got = _run(0 | tile << 4 | split << 8 | kind << 11 | 1 << 12, N, H, W, Ci, Co, x, None, w, b,
+                           size=N * H * W * Co * 5 // 4)
+                full = got[:N * H * W * Co].reshape(N, H, W, Co)
+                assert np.abs(full - ref_f).max() < 1e-5 * scale * 10 + 0.01 * np.abs(ref_f).max(), (tile, kind)
+                v = full.astype(np.float32)
+                want = cr.bf16(np.float32(0.25) * ((v[:, 0::2, 0::2] + v[:, 0::2, 1::2]) +
+                                                   (v[:, 1::2, 0::2] + v[:, 1::2, 1::2])))
+                assert np.array_equal(got[N * H * W * Co:].reshape(want.shape), want), ("pool", tile, kind, split)
     # weight gradient
     got = _run(2, N, H, W, Ci, Co, x, g, None, None).reshape(9, Co, Ci)
     ref = cr.conv3x3_wgrad(x, g)
