@@ -380,7 +380,9 @@ int lbm_velocity_skill(lbm_grid *g, double out[2]);
  * over blocks (0 = chosen by size, or 1, 2, 4; at most the number of 32-channel
  * input chunks), bit 11 the kernel that stages the weights in LDS (block shapes
  * with at most 64 output channels; a forced shape otherwise runs the kernel
- * that reads weights per wave).  Bit 12 with mode 0: a bf16 output and its
+ * that reads weights per wave), bit 13 that kernel with 64-channel input
+ * chunks (block shape (1,1,1) only, input channels a multiple of 64).  Bit 12
+ * with mode 0: a bf16 output and its
  * 2 x 2 average pool, returned as out[0, N H W Cout) and out[N H W Cout,
  * + N H W Cout / 4) (H even). */
 int lbm_conv3x3_test(int mode, int n, int h, int w, int cin, int cout, const float *in, const float *g,

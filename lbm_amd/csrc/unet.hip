@@ -418,18 +418,23 @@ __global__ __launch_bounds__(256) void k_conv3_lds(ConvIn src, const uint16_t *_
 // kernel above once a layer has several input chunks.  Both tables are double
 // buffered and filled from registers loaded during the previous chunk's
 // products.  Block shapes with 32 * WN * NI <= 64 output channels only (LDS).
-template <int MI, int NI, int WN>
+// KC input channels per chunk: 32, or 64 (fewer, longer chunks: twice the
+// products per wait for the next chunk's loads; 32-channel blocks of four rows
+// only, 141 KB of LDS).
+template <int MI, int NI, int WN, int KC = 32>
 __global__ __launch_bounds__(256) void k_conv3_lw(ConvIn src, const uint16_t *__restrict__ w,
                                                   const float *__restrict__ bias, void *__restrict__ out,
                                                   float *__restrict__ part, uint16_t *__restrict__ pout, int N, int H,
                                                    int W, int Cout,
                                                   int co_store, int relu, int out_f32) {
-    constexpr int WM = 4 / WN, TH = WM * MI, HR = TH + 2, HC = 34, PS = 40;
+    constexpr int WM = 4 / WN, TH = WM * MI, HR = TH + 2, HC = 34;
+    constexpr int PS = KC + 8;           // element stride of a pixel / a weight row: 16 bytes of padding
+    constexpr int PPX = KC / 8;          // 16-byte pieces per pixel / weight row
     constexpr int CB = 32 * NI * WN;     // output channels of the block
-    static_assert(CB <= 64, "weight table too large for LDS");
-    constexpr int NPIECE = HR * HC * 4;  // 16-byte pieces of one chunk's halo
+    static_assert(KC == 32 ? CB <= 64 : (KC == 64 && CB == 32 && TH <= 4), "tables too large for LDS");
+    constexpr int NPIECE = HR * HC * PPX;  // 16-byte pieces of one chunk's halo
     constexpr int PER_T = (NPIECE + 255) / 256;
-    constexpr int WPIECE = 9 * CB * 4;   // 16-byte pieces of one chunk's weights
+    constexpr int WPIECE = 9 * CB * PPX;   // 16-byte pieces of one chunk's weights
     constexpr int WPER_T = (WPIECE + 255) / 256;
     constexpr int HALO = HR * HC * PS, WTAB = 9 * CB * PS;
     __shared__ __align__(16) uint16_t lds[2][HALO + WTAB];
@@ -446,18 +451,18 @@ __global__ __launch_bounds__(256) void k_conv3_lw(ConvIn src, const uint16_t *__
     const int half = lane >> 5, l32 = lane & 31;
     const int cb0 = blockIdx.y * CB;
     const int up = src.a_up ? 1 : 0, Ha = H >> up, Wa = W >> up;
-    const int nck = Cin >> 5;
+    const int nck = Cin / KC;
     const int ck0 = (int)(blockIdx.z * nck / gridDim.z), ck1 = (int)((blockIdx.z + 1) * nck / gridDim.z);
 
     u16x8 stage[PER_T], wstage[WPER_T];
     auto gload = [&](int ck) {
-        const int c0 = ck << 5;
+        const int c0 = ck * KC;
 #pragma unroll
         for (int t = 0; t < PER_T; ++t) {
             const int q = threadIdx.x + 256 * t;
             stage[t] = u16x8{};
             if (q < NPIECE) {
-                const int pi = q >> 2, c = c0 + ((q & 3) << 3);
+                const int pi = q / PPX, c = c0 + (q % PPX) * 8;  // a piece lies wholly in a or in b
                 const int y = y0 - 1 + pi / HC, x = x0 - 1 + pi % HC;
                 if (y >= 0 && y < H && x >= 0 && x < W) {
                     const uint16_t *ptr = c < src.ca
@@ -467,13 +472,13 @@ __global__ __launch_bounds__(256) void k_conv3_lw(ConvIn src, const uint16_t *__
                 }
             }
         }
-        // weights: piece q = (tap * CB + cout) * 4 + 16-byte piece of the chunk
+        // weights: piece q = (tap * CB + cout) * PPX + 16-byte piece of the chunk
 #pragma unroll
         for (int t = 0; t < WPER_T; ++t) {
             const int q = threadIdx.x + 256 * t;
             if (q < WPIECE) {
-                const int row = q >> 2, tap = row / CB, cl = row % CB;
-                wstage[t] = *(const u16x8 *)(w + ((size_t)tap * Cout + cb0 + cl) * Cin + c0 + ((q & 3) << 3));
+                const int row = q / PPX, tap = row / CB, cl = row % CB;
+                wstage[t] = *(const u16x8 *)(w + ((size_t)tap * Cout + cb0 + cl) * Cin + c0 + (q % PPX) * 8);
             }
         }
     };
@@ -481,12 +486,12 @@ __global__ __launch_bounds__(256) void k_conv3_lw(ConvIn src, const uint16_t *__
 #pragma unroll
         for (int t = 0; t < PER_T; ++t) {
             const int q = threadIdx.x + 256 * t;
-            if (q < NPIECE) *(u16x8 *)&lds[buf][(q >> 2) * PS + ((q & 3) << 3)] = stage[t];
+            if (q < NPIECE) *(u16x8 *)&lds[buf][(q / PPX) * PS + (q % PPX) * 8] = stage[t];
         }
 #pragma unroll
         for (int t = 0; t < WPER_T; ++t) {
             const int q = threadIdx.x + 256 * t;
-            if (q < WPIECE) *(u16x8 *)&lds[buf][HALO + (q >> 2) * PS + ((q & 3) << 3)] = wstage[t];
+            if (q < WPIECE) *(u16x8 *)&lds[buf][HALO + (q / PPX) * PS + (q % PPX) * 8] = wstage[t];
         }
     };
     f32x16 acc[MI][NI];
@@ -499,7 +504,7 @@ __global__ __launch_bounds__(256) void k_conv3_lw(ConvIn src, const uint16_t *__
         for (int tap = 0; tap < 9; ++tap) {
             const int dy = tap / 3, dx = tap % 3;
 #pragma unroll
-            for (int s2 = 0; s2 < 2; ++s2) {
+            for (int s2 = 0; s2 < KC / 16; ++s2) {
                 u16x8 b[NI];
 #pragma unroll
                 for (int j = 0; j < NI; ++j)
@@ -852,11 +857,13 @@ ConvPlan conv3_plan(int N, int H, int W, int Cin, int Cout) {
     // channels (one input chunk, where the halo-only kernel keeps twice the
     // blocks resident); K split while the blocks cover fewer than half the
     // CUs (at 128..255 blocks the split's extra pass costs more than it gains).
+    // Where those blocks leave at most one per CU anyway, 64-channel chunks.
     if (!(Cin == 32 && Cout == 32)) {
+        const int kind = Cin % 64 == 0 && conv3_blocks(0, N, H, W, Cout) <= 256 ? 2 : 1;
         int split = 1;
-        const int nck = Cin / 32;
+        const int nck = Cin / (kind == 2 ? 64 : 32);
         while (split < 4 && 2 * split <= nck && conv3_blocks(0, N, H, W, Cout) * split < 128) split *= 2;
-        return {0, split, 1};
+        return {0, split, kind};
     }
     // preference lists, indices into kTiles3
     static const int p32a[] = {1, 0, 2};           // Cout 32, Cin 32
@@ -947,7 +954,7 @@ void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const 
     const int t = pl.tile;
     ucheck(t >= 0 && t < 13 && Cout % (32 * kTiles3[t][1] * kTiles3[t][2]) == 0, "unsupported block shape");
     ucheck(pl.split == 1 || pl.split == 2 || pl.split == 4, "unsupported split");
-    while (pl.split > Cin / 32) pl.split /= 2;  // every split gets at least one input chunk
+    while (pl.split > Cin / (pl.kind == 2 ? 64 : 32)) pl.split /= 2;  // every split gets at least one input chunk
     const bool split = pl.split > 1;
     ucheck(!split || (ws && ws_elems >= (size_t)pl.split * P * Cout), "convolution split workspace too small");
     const dim3 grid((unsigned)conv3_blocks(t, N, H, W, Cout) / (Cout / (32 * kTiles3[t][1] * kTiles3[t][2])),
@@ -955,6 +962,7 @@ void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const 
     const int r = relu ? 1 : 0, f = out_f32 ? 1 : 0;
     const int kind = pl.kind;
     ucheck(kind == 0 || 32 * kTiles3[t][1] * kTiles3[t][2] <= 64, "block shape too wide for LDS weights");
+    ucheck(kind != 2 || (t == 0 && Cin % 64 == 0), "64-channel chunks need the (1,1,1) block and Cin % 64 == 0");
     // the 2 x 2 average pool of the output rides in the epilogue when the block
     // holds row pairs and the output is final (no K split); otherwise k_pool2
     ucheck(!pool_out || (!out_f32 && co_store == Cout && H % 2 == 0), "pooled output needs a full bf16 output");
@@ -962,7 +970,10 @@ void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const 
     uint16_t *pout = fuse_pool ? pool_out : nullptr;
 #define L3(MI, NI, WN)                                                                                             \
     do {                                                                                                           \
-        if (kind == 1 && (MI) * 0 + 32 * (NI) * (WN) <= 64)                                                        \
+        if (kind == 2)                                                                                             \
+            hipLaunchKernelGGL((k_conv3_lw<1, 1, 1, 64>), grid, dim3(256), 0, s, in, w, bias, out, ws, pout, N, H, \
+                               W, Cout, co_store, r, f);                                                           \
+        else if (kind == 1 && 32 * (NI) * (WN) <= 64)                                                              \
             hipLaunchKernelGGL((k_conv3_lw<MI, NI, (32 * NI * WN <= 64 ? WN : 1)>), grid, dim3(256), 0, s, in, w,   \
                                bias, out, ws, pout, N, H, W, Cout, co_store, r, f);                                \
         else                                                                                                       \
@@ -1566,9 +1577,9 @@ int lbm_conv3x3_test(int mode, int N, int H, int W, int Cin, int Cout, const flo
         ConvTile force;
         force.tile = (mode >> 4) & 15;
         force.split = (mode >> 8) & 7;
-        force.kind = (mode >> 11) & 1 ? 1 : -1;
+        force.kind = (mode >> 13) & 1 ? 2 : (mode >> 11) & 1 ? 1 : -1;
         const bool pooled = (mode >> 12) & 1;
-        ucheck(force.tile <= 13 && (mode >> 13) == 0, "mode");
+        ucheck(force.tile <= 13 && (mode >> 14) == 0, "mode");
         ucheck(!pooled || ((mode & 3) == 0 && H % 2 == 0), "mode: pooled output needs the forward and an even height");
         ucheck(force.split == 0 || force.split == 1 || force.split == 2 || force.split == 4, "mode: split");
         mode = op;

@@ -94,16 +94,18 @@ def test_mfma_conv_matches_oracle(N, H, W, Ci, Co):
     # kind 1: the kernel that stages the weights in LDS (block shapes with <= 64 output channels)
     cases = [(tile, 0) for tile in range(len(SHAPES) + 1)]
     cases += [(tile, 1) for tile in range(1, len(SHAPES) + 1) if 32 * SHAPES[tile - 1][1] * SHAPES[tile - 1][2] <= 64]
+    # kind 2: the same with 64-channel input chunks, (1,1,1) blocks only
+    cases += [(1, 2)]
     for tile, kind in cases:
         nw = 32 * SHAPES[tile - 1][1] * SHAPES[tile - 1][2] if tile else 32
         for split in (0, 1, 2, 4):
-            mode = tile << 4 | split << 8 | kind << 11
+            mode = tile << 4 | split << 8 | (kind == 1) << 11 | (kind == 2) << 13
             # forward: bf16 inputs are exact, accumulation in f32
-            if Co % nw == 0:
+            if Co % nw == 0 and (kind != 2 or Ci % 64 == 0):
                 got = _run(0 | mode, N, H, W, Ci, Co, x, None, w, b).reshape(N, H, W, Co)
                 assert np.abs(got - ref_f).max() < 1e-5 * scale * 10, (tile, split, kind)
             # input gradient: the same kernel with Cin and Cout exchanged
-            if Ci % nw == 0:
+            if Ci % nw == 0 and (kind != 2 or Co % 64 == 0):
                 got = _run(1 | mode, N, H, W, Ci, Co, g, None, w, None).reshape(N, H, W, Ci)
                 assert np.abs(got - ref_d).max() < 1e-4 * np.sqrt(9 * Co), (tile, split, kind)
     # the 2 x 2 average pool in the convolution's epilogue equals k_pool2's arithmetic on the
@@ -112,8 +114,11 @@ def test_mfma_conv_matches_oracle(N, H, W, Ci, Co):
         # every block shape: those without row pairs (and K splits) take the separate pool pass
         pool_cases = [(t, k) for t, k in cases if not t or Co % (32 * SHAPES[t - 1][1] * SHAPES[t - 1][2]) == 0]
         for tile, kind in pool_cases:
+            if kind == 2 and Ci % 64:
+                continue
             for split in (0, 2):
-                got = _run(0 | tile << 4 | split << 8 | kind << 11 | 1 << 12, N, H, W, Ci, Co, x, None, w, b,
+                got = _run(0 | tile << 4 | split << 8 | (kind == 1) << 11 | (kind == 2) << 13 | 1 << 12,
+                           N, H, W, Ci, Co, x, None, w, b,
                            size=N * H * W * Co * 5 // 4)
                 full = got[:N * H * W * Co].reshape(N, H, W, Co)
                 assert np.abs(full - ref_f).max() < 1e-5 * scale * 10 + 0.01 * np.abs(ref_f).max(), (tile, kind)
