@@ -12,9 +12,13 @@ plain LBM runs the same steps; the residual is the velocity rel-L2 between
 the two (FP64 sums on the device) -- the cycle's true error -- and the run
 continues from the plain state.  A residual above ``threshold`` (or a
 non-finite prediction) is a fallback: the network is not trusted, and the
-cycles up to the next check run the plain LBM.  Measured one-cycle errors on
-the benchmark flows: the trained network 0.35-1.2%, the network-free method
-(change predicted as zero) 4.5-7.5%.
+cycles up to the next check run the plain LBM.  With ``max_check_every``
+above ``check_every`` the interval backs off: a check passed with a residual
+at most half the threshold doubles the interval (up to ``max_check_every``),
+one passed closer to the threshold keeps it, a fallback resets it to
+``check_every``.  Measured one-cycle errors on the benchmark flows: the
+trained network 0.35-1.2%, the network-free method (change predicted as zero)
+4.5-7.5%.
 
 Time left over at the end (fewer than a cycle's steps) runs plain.
 ``compare`` runs both from the same state and reports the velocity error and
@@ -44,13 +48,16 @@ class RolloutStats:
 
 class Rollout:
     def __init__(self, grid, net, jump: int = 64, correctors: int = 8, threshold: float = 0.5,
-                 check_every: int = 8, s_rho: float = S_RHO, s_u: float = S_U, d_rho: float = D_RHO,
-                 d_u: float = D_U):
+                 check_every: int = 8, max_check_every: int | None = None, s_rho: float = S_RHO,
+                 s_u: float = S_U, d_rho: float = D_RHO, d_u: float = D_U):
         if check_every < 0:
             raise ValueError("check_every must be >= 0 (0: never check, every cycle jumps)")
+        if max_check_every is not None and max_check_every < check_every:
+            raise ValueError("max_check_every must be >= check_every")
         self.grid, self.net = grid, net
         self.jump, self.correctors, self.threshold = jump, correctors, threshold
         self.check_every = check_every
+        self.max_check_every = check_every if max_check_every is None else max_check_every
         self.scales = (s_rho, s_u, d_rho, d_u)
 
     @property
@@ -62,8 +69,9 @@ class Rollout:
         g = self.grid
         per = self.cycle_steps
         trusted = self.check_every == 0
+        interval, next_check = self.check_every, 0
         while steps - st.steps >= per:
-            if self.check_every and st.cycles % self.check_every == 0:
+            if self.check_every and st.cycles == next_check:
                 g.velocity_mark(0)  # the cycle's start: the persistence forecast
                 g.snapshot()
                 g.surrogate_jump(self.net, self.jump, *self.scales)
@@ -78,6 +86,10 @@ class Rollout:
                 st.checks += 1
                 if not trusted:
                     st.fallbacks += 1
+                    interval = self.check_every
+                elif r <= 0.5 * self.threshold:
+                    interval = min(2 * interval, self.max_check_every)
+                next_check = st.cycles + interval
             elif trusted:
                 g.surrogate_jump(self.net, self.jump, *self.scales)
                 g.step(self.correctors)
@@ -168,6 +180,8 @@ def main(argv=None) -> int:
     ap.add_argument("--spinup", type=int, default=200)
     ap.add_argument("--cycles", type=int, default=20)
     ap.add_argument("--threshold", type=float, default=0.5)
+    ap.add_argument("--check-every", type=int, default=8)
+    ap.add_argument("--max-check-every", type=int, default=None, help="back off up to this interval (default: none)")
     ap.add_argument("--weights", default=os.path.join(os.path.dirname(__file__), "data", "surrogate_f2.npy"))
     a = ap.parse_args(argv)
     g = LBM(1.0 / a.tau, a.nx, a.ny)
@@ -176,10 +190,11 @@ def main(argv=None) -> int:
     g.step(a.spinup)
     net = UNet(1, a.ny // 2, a.nx // 2)
     net.set_params(load_params(a.weights))
-    ro = Rollout(g, net, threshold=a.threshold)
+    ro = Rollout(g, net, threshold=a.threshold, check_every=a.check_every, max_check_every=a.max_check_every)
     st = ro.run(ro.cycle_steps * a.cycles)
     g.sync()
-    print(f"cycles {st.cycles} jumps {st.jumps} fallbacks {st.fallbacks} steps {g.get_compute_num()}", flush=True)
+    print(f"cycles {st.cycles} jumps {st.jumps} checks {st.checks} fallbacks {st.fallbacks} "
+          f"steps {g.get_compute_num()}", flush=True)
     net.close()
     g.close()
     return 0
