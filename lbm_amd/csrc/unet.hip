@@ -944,6 +944,12 @@ void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const 
         HIPCHK(hipGetLastError());
         return;
     }
+    // the 3x3 epilogues move four channels at a time (float4 bias and partial
+    // sums, 8- or 16-byte output and pooled stores)
+    auto aligned = [](const void *q, uintptr_t a) { return ((uintptr_t)q & (a - 1)) == 0; };
+    ucheck((!bias || aligned(bias, 16)) && aligned(out, out_f32 ? 16 : 8) && (!ws || aligned(ws, 16)) &&
+               (!pool_out || aligned(pool_out, 8)),
+           "convolution buffers must be 16-byte aligned (8 for bf16 outputs)");
     ConvPlan pl = conv3_plan(N, H, W, Cin, Cout);
     if (force.tile) {  // a forced block shape runs on the halo-only kernel unless the kind is forced too
         pl.tile = force.tile - 1;

@@ -42,9 +42,18 @@ def _conv(x, layer, rnd):
     return y
 
 
-def forward(params, xin, round_bf16=True):
-    """xin (N, H, W, c <= 32) -> (N, H, W, 3); the input is zero-padded to 32 channels."""
+def forward(params, xin, round_bf16=True, mutate=None):
+    """xin (N, H, W, c <= 32) -> (N, H, W, 3); the input is zero-padded to 32 channels.
+
+    ``mutate`` plants one wiring mistake, for tests that must tell a correct
+    network from a miswired one: "skip_order" (skip tensor first in each
+    concatenation), "skip_shift" (skip channels rotated by one), "up_shift"
+    (upsampled tensors shifted by one column), "tap_flip" (the 3x3 taps of
+    every convolution in reverse order)."""
+    assert mutate in (None, "skip_order", "skip_shift", "up_shift", "tap_flip")
     L = unpack(np.asarray(params, np.float64))
+    if mutate == "tap_flip":
+        L = [(ks, w[::-1] if ks == 3 else w, b, relu) for ks, w, b, relu in L]
     r = (lambda a: bf16(a)) if round_bf16 else (lambda a: a)
     N, H, W, c = xin.shape
     x = np.zeros((N, H, W, 32))
@@ -58,15 +67,21 @@ def forward(params, xin, round_bf16=True):
         return r(0.25 * (t[:, 0::2, 0::2] + t[:, 0::2, 1::2] + t[:, 1::2, 0::2] + t[:, 1::2, 1::2]))
 
     def up(t):
-        return t.repeat(2, axis=1).repeat(2, axis=2)
+        u = t.repeat(2, axis=1).repeat(2, axis=2)
+        return np.roll(u, 1, axis=2) if mutate == "up_shift" else u
+
+    def cat(a, skip):
+        if mutate == "skip_shift":
+            skip = np.roll(skip, 1, axis=-1)
+        return np.concatenate([skip, a] if mutate == "skip_order" else [a, skip], axis=-1)
 
     e1 = c(1, c(0, x))
     e2 = c(3, c(2, pool(e1)))
     e3 = c(5, c(4, pool(e2)))
     e4 = c(7, c(6, pool(e3)))
-    d3 = c(10, c(9, np.concatenate([c(8, up(e4)), e3], axis=-1)))
-    d2 = c(13, c(12, np.concatenate([c(11, up(d3)), e2], axis=-1)))
-    d1 = c(16, c(15, np.concatenate([c(14, up(d2)), e1], axis=-1)))
+    d3 = c(10, c(9, cat(c(8, up(e4)), e3)))
+    d2 = c(13, c(12, cat(c(11, up(d3)), e2)))
+    d1 = c(16, c(15, cat(c(14, up(d2)), e1)))
     out = _conv(d1, L[17], round_bf16)
     return out[..., :3]
 
