@@ -703,6 +703,23 @@ struct lbm_grid {
     }
 
     // moments of the current state into `now` (optionally with the FP64 reduction)
+    // flush() and moments_now(false) in one pass where the stored S-form state
+    // decodes to exactly the pulled values the moments are taken from (f32,
+    // f - w).  F16E re-rounds a bounced population into its new population's
+    // shift when it stores it, so its moments come from the stored state in a
+    // pass of their own.
+    void flush_with_moments() {
+        ensure_now();
+        if (pform && precision != LBM_PRECISION_F16E) {
+            cell<true, OP_STORE_S | OP_MOMENTS>(f[cur], f[1 - cur], now, nullptr);
+            cur = 1 - cur;
+            pform = false;
+            return;
+        }
+        flush();
+        moments_now(false);
+    }
+
     void moments_now(bool reduce) {
         ensure_now();
         if (reduce) ensure_partials();
@@ -1285,8 +1302,7 @@ int lbm_surrogate_jump(lbm_grid *g, lbm_unet *net, uint32_t steps, float s_rho, 
         const bool coarse = b == 1 && 2 * h == (int)g->Y && 2 * w == (int)g->X;
         check(fine || coarse, "network shape must be (1, ny, nx) or (1, ny / 2, nx / 2)");
         check(unet_device(net) == g->dev, "network and grid on different devices");
-        g->flush();  // S-form: buf[cur] is the state to predict from and to overwrite
-        g->moments_now(false);
+        g->flush_with_moments();  // S-form: buf[cur] is the state to predict from and to overwrite
         unet_begin_use(net, g->stream);  // after any use of the network on another stream
         const dim3 pgrid((w + 255) / 256, h);
         if (fine)
