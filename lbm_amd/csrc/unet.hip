@@ -285,6 +285,67 @@ __device__ __forceinline__ void conv_pool_tile(const f32x16 (&acc)[MI][NI], cons
     }
 }
 
+// A 1x1 convolution applied to a 3x3 convolution's output inside its
+// epilogue (the network's head): out[p * cout + o] = b[o] + sum_c w[o][c] y[p][c]
+// in f32 for o < store rounded up to a multiple of 4 (outputs are stored four
+// at a time), y the stored (bf16) output of a 32-channel layer and w
+// [cout = 32][32] bf16.  Null w: none.
+struct ConvHead {
+    const uint16_t *w = nullptr;
+    const float *b = nullptr;
+    float *out = nullptr;
+    int cout = 0, store = 0;
+};
+
+// One more product per row of the wave, on values the wave already holds: the
+// B operand is the lane's own 16 channels of its pixel (channels 8 g + 4 half
+// + e, two groups per k-step), the A operand the head's weights in the same
+// channel order, so the K sum runs over all 32 channels; the result is
+// [head output][pixel] like the 3x3 products'.
+template <int MI>
+__device__ __forceinline__ void conv_head_tile(const f32x16 (&acc)[MI][1], const float *__restrict__ bias, int relu,
+                                               const ConvHead &hd, int H, int W, long long n, int x0, int y_first,
+                                               int lane) {
+    const int half = lane >> 5, l32 = lane & 31;
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+        const int y = y_first + i;
+        f32x16 d = f32x16{};
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+            u16x8 bop, aop;
+#pragma unroll
+            for (int gg = 0; gg < 2; ++gg) {
+                const int g = 2 * s2 + gg, c = 8 * g + 4 * half;
+                const float4 bv = bias ? *(const float4 *)&bias[c] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                float v[4] = {acc[i][0][4 * g] + bv.x, acc[i][0][4 * g + 1] + bv.y, acc[i][0][4 * g + 2] + bv.z,
+                              acc[i][0][4 * g + 3] + bv.w};
+                const uint2 wv = *(const uint2 *)&hd.w[l32 * 32 + c];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) bop[4 * gg + e] = f2bf(relu ? fmaxf(v[e], 0.0f) : v[e]);
+                aop[4 * gg] = (uint16_t)wv.x;
+                aop[4 * gg + 1] = (uint16_t)(wv.x >> 16);
+                aop[4 * gg + 2] = (uint16_t)wv.y;
+                aop[4 * gg + 3] = (uint16_t)(wv.y >> 16);
+            }
+            d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, aop), __builtin_bit_cast(bf16x8, bop),
+                                                        d, 0, 0, 0);
+        }
+        if (y < H) {
+            const long long p = (n * H + y) * W + x0 + l32;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int o = 8 * g + 4 * half;  // four consecutive outputs, stored as one vector
+                if (o < hd.store) {
+                    const float4 hb = *(const float4 *)&hd.b[o];
+                    *(float4 *)&hd.out[p * hd.cout + o] =
+                        make_float4(d[4 * g] + hb.x, d[4 * g + 1] + hb.y, d[4 * g + 2] + hb.z, d[4 * g + 3] + hb.w);
+                }
+            }
+        }
+    }
+}
+
 // LDS-staged 3x3 convolution.  A block computes TH = (4 / WN) * MI rows x 32
 // columns of output pixels of one image for 32 * WN * NI output channels; its
 // four waves form a (4 / WN) x WN grid, each computing MI rows x NI tiles of 32
@@ -301,7 +362,7 @@ __device__ __forceinline__ void conv_pool_tile(const f32x16 (&acc)[MI][NI], cons
 template <int MI, int NI, int WN>
 __global__ __launch_bounds__(256) void k_conv3_lds(ConvIn src, const uint16_t *__restrict__ w,
                                                    const float *__restrict__ bias, void *__restrict__ out,
-                                                   float *__restrict__ part, uint16_t *__restrict__ pout, int N, int H,
+                                                   float *__restrict__ part, uint16_t *__restrict__ pout, ConvHead hd, int N, int H,
                                                    int W, int Cout,
                                                    int co_store, int relu, int out_f32) {
     constexpr int WM = 4 / WN, TH = WM * MI, HR = TH + 2, HC = 34, PS = 40;
@@ -407,6 +468,8 @@ __global__ __launch_bounds__(256) void k_conv3_lds(ConvIn src, const uint16_t *_
                             half, l32);
     if constexpr (MI == 2 || (MI == 1 && WN == 1))
         if (pout) conv_pool_tile<MI, NI, WN>(acc, bias, pout, &lds[0][0], H, W, Cout, relu, n, x0, y0 + wm * MI, wm, co0, lane);
+    if constexpr (NI == 1 && WN == 1)
+        if (hd.w) conv_head_tile<MI>(acc, bias, relu, hd, H, W, n, x0, y0 + wm * MI, lane);
 }
 
 // The same convolution with the weights staged in LDS too.  Per chunk the
@@ -915,14 +978,14 @@ size_t conv_split_elems(int ks, long long P, int N, int H, int W, int Cin, int C
 // the K split is taken).
 void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const float *bias, void *out, int N, int H,
               int W, int Cout, bool relu, bool out_f32, int co_store, float *ws, size_t ws_elems,
-              ConvTile force = ConvTile{}, uint16_t *pool_out = nullptr) {
+              ConvTile force = ConvTile{}, uint16_t *pool_out = nullptr, ConvHead head = ConvHead{}) {
     const long long P = (long long)N * H * W;
     const int Cin = in.ca + in.cb;
     ucheck(W % 32 == 0 && in.ca % 32 == 0 && in.cb % 32 == 0 && Cin > 0 && Cout % 32 == 0,
            "convolution shape: W, channel counts must be multiples of 32");
     ucheck(!in.a_up || H % 2 == 0, "upsampled input needs an even height");
     if (ks == 1) {
-        ucheck(!in.a_up && in.cb == 0 && !pool_out, "1x1 convolutions take one plain input, no pooled output");
+        ucheck(!in.a_up && in.cb == 0 && !pool_out && !head.w, "1x1 convolutions take one plain input only");
         int mi = 1, ni = 1;
         if (P % 64 == 0 && P / 64 * (Cout / 32) >= 4096) mi = 2;
         else if (Cout % 64 == 0 && P / 32 * (Cout / 64) >= 4096) ni = 2;
@@ -974,6 +1037,13 @@ void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const 
     ucheck(!pool_out || (!out_f32 && co_store == Cout && H % 2 == 0), "pooled output needs a full bf16 output");
     const bool fuse_pool = pool_out && !split && (kTiles3[t][0] == 2 || (kTiles3[t][0] == 1 && kTiles3[t][2] == 1));
     uint16_t *pout = fuse_pool ? pool_out : nullptr;
+    // a 1x1 head rides in the epilogue of the per-wave-weights kernel when one
+    // wave holds all 32 output channels of its pixels; otherwise its own pass
+    ucheck(!head.w || (Cout == 32 && !out_f32 && co_store == Cout && head.b && head.out && head.store <= 32 &&
+                       head.cout == 32 && aligned(head.w, 8) && aligned(head.b, 16) && aligned(head.out, 16)),
+           "head: a bf16 32-channel layer, 32 x 32 weights, 32 f32 outputs per pixel");
+    const bool fuse_head = head.w && kind == 0 && !split && kTiles3[t][1] == 1 && kTiles3[t][2] == 1;
+    const ConvHead hd = fuse_head ? head : ConvHead{};
 #define L3(MI, NI, WN)                                                                                             \
     do {                                                                                                           \
         if (kind == 2)                                                                                             \
@@ -983,8 +1053,8 @@ void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const 
             hipLaunchKernelGGL((k_conv3_lw<MI, NI, (32 * NI * WN <= 64 ? WN : 1)>), grid, dim3(256), 0, s, in, w,   \
                                bias, out, ws, pout, N, H, W, Cout, co_store, r, f);                                \
         else                                                                                                       \
-            hipLaunchKernelGGL((k_conv3_lds<MI, NI, WN>), grid, dim3(256), 0, s, in, w, bias, out, ws, pout, N, H, \
-                               W, Cout, co_store, r, f);                                                           \
+            hipLaunchKernelGGL((k_conv3_lds<MI, NI, WN>), grid, dim3(256), 0, s, in, w, bias, out, ws, pout, hd, N, \
+                               H, W, Cout, co_store, r, f);                                                        \
     } while (0)
     switch (t) {
         case 0: L3(1, 1, 1); break;
@@ -1019,6 +1089,9 @@ void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const 
                            pool_out, N, H, W, Cout);
         HIPCHK(hipGetLastError());
     }
+    if (head.w && !fuse_head)
+        conv_fwd(s, 1, one_input((const uint16_t *)out, 32), head.w, head.b, head.out, N, H, W, head.cout, false, true,
+                 head.store, ws, ws_elems);
 }
 
 void conv_wgrad(hipStream_t s, int ks, const ConvIn &in, const uint16_t *g, float *dw, int N, int H, int W,
@@ -1280,10 +1353,17 @@ struct lbm_unet {
         fwd_conv(13, d2a, d2);
         fwd_conv(14, d2, d1u);
         conv(15, cat_up(d1u, e1), e1.H, e1.W, d1a.a);
-        fwd_conv(16, d1a, d1);
-        const Conv &h = convs[17];
-        conv_fwd(stream, h.ks, one_input(d1.a, d1.C), wb + h.wboff, w32 + h.boff, out32, B, H, W, h.cout, false, true,
-                 3, ws, ws_elems);
+        // the last 3x3 layer carries the 1x1 head (three f32 outputs per pixel) in its epilogue
+        const Conv &c16 = convs[16], &h = convs[17];
+        ucheck(h.ks == 1 && h.cin == 32 && c16.cout == 32, "head layer");
+        ConvHead hd;
+        hd.w = wb + h.wboff;
+        hd.b = w32 + h.boff;
+        hd.out = out32;
+        hd.cout = h.cout;
+        hd.store = 3;
+        conv_fwd(stream, c16.ks, one_input(d1a.a, d1a.C), wb + c16.wboff, w32 + c16.boff, d1.a, B, H, W, c16.cout,
+                 c16.relu, false, c16.cout, ws, ws_elems, ConvTile{}, nullptr, hd);
         HIPCHK(hipGetLastError());
     }
 
