@@ -50,7 +50,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_interior(const typename St::T 
                                                            float *__restrict__ mom, double *__restrict__ partials,
                                                            int chunked) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    // wave-uniform list positions: the list entry and the domain's table entry
+    // then come through scalar loads, once per wave instead of once per lane
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
     const uint32_t waves = gridDim.x * (kBlock / 64);
     uint32_t t = wave, stride = waves, end = n;
     if (chunked) {  // each wave a contiguous run of the list
@@ -81,8 +83,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_boundary(const typename St::T 
                                                            float *__restrict__ mom, double *__restrict__ partials) {
     const int lane = threadIdx.x & 63;
     const uint32_t waves = gridDim.x * (kBlock / 64);
+    // wave-uniform list position (scalar loads of the list and table entries)
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (uint32_t t = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); t < n; t += waves) {
+    for (uint32_t t = wave; t < n; t += waves) {
         const uint2 e = list[t];
         const DomainDev d = table[e.x];
         const int y = (int)(e.y >> 16);
