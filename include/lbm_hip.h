@@ -251,7 +251,7 @@ typedef struct lbm_domain {
     float u_in;   /* uniform x velocity of the initial / inflow equilibrium */
 } lbm_domain;
 
-#define LBM_BATCH_NUM_VARIANTS 6
+#define LBM_BATCH_NUM_VARIANTS 8
 
 int lbm_batch_create(const lbm_domain *domains, uint32_t n, int32_t precision, int32_t boundary,
                      int32_t device, lbm_batch **out);
@@ -269,7 +269,13 @@ int lbm_batch_step_count(const lbm_batch *b, uint64_t *n);
 int lbm_batch_info(lbm_batch *b, uint64_t *cells, uint32_t *interior_tiles, uint32_t *boundary_segments);
 /* hipGraph replay on (default) or off */
 int lbm_batch_set_graph(lbm_batch *b, int enable);
-/* interior-kernel tile variants (cells per thread, list order); autotune as lbm_autotune */
+/* Batch variants: 0..5 the interior kernel's tile (cells per thread 1, 2, 4
+ * -- 2, 4, 8 for binary16 -- each in list order or chunked); 6, 7 two steps
+ * per launch for every domain (row chunks of 8 / 16), which needs two more
+ * state sets: lbm_batch_set_variant allocates them (and fails, changing
+ * nothing, when it cannot) and frees them when another variant is selected.
+ * All variants give the same bits.  lbm_batch_autotune as lbm_autotune (+inf
+ * for a two-step variant without memory or with steps < 3). */
 int lbm_batch_set_variant(lbm_batch *b, int variant);
 int lbm_batch_get_variant(const lbm_batch *b, int *variant);
 int lbm_batch_autotune(lbm_batch *b, uint32_t steps, int *best, float *ms);

@@ -19,7 +19,7 @@ LBM_ENONFINITE = -4
 LBM_ENODEV = -5
 
 NUM_VARIANTS = 10
-BATCH_NUM_VARIANTS = 6
+BATCH_NUM_VARIANTS = 8
 
 PRECISION_F32 = 0
 PRECISION_F16S = 1

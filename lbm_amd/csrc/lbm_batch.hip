@@ -31,13 +31,15 @@
 
 #include "lbm_common.h"
 #include "tune_cache.h"
+#include "pair.h"
 
 namespace {
 
 struct DomainDev {
-    long long f_off;    // element offset of the domain in each parity set
-    long long aux_off;  // element offset in the mask / flags planes (and moment planes)
-    long long seg_off;  // offset in the segment plane
+    long long f_off;     // element offset of the domain in each parity set
+    long long aux_off;   // element offset in the mask / flags planes (and moment planes)
+    long long seg_off;   // offset in the segment plane
+    long long col0_off;  // element offset of the domain's 9 x Y column-0 buffer (two-step variants)
     Geom g;
 };
 
@@ -102,6 +104,37 @@ __global__ __launch_bounds__(kBlock) void k_batch_boundary(const typename St::T 
     if (OP & OP_REDUCE) block_sums(s0, s1, s2, partials, blockIdx.x);
 }
 
+// Two steps per launch for every domain (the single grid's two-step scheme,
+// pair.h): one block per work item (domain, 254-column strip, row chunk) of
+// the pair list, blocks of one XCD on one contiguous run of it.  A, B: the
+// current sets; A2, B2: the spare sets that receive C_{n+2} and C_{n+1}.
+template <class St, int MODE>
+__global__ __launch_bounds__(kBlock) void k_batch_pair(const typename St::T *A, const typename St::T *B,
+                                                       typename St::T *__restrict__ A2, typename St::T *__restrict__ B2,
+                                                       const typename St::T *__restrict__ col0,
+                                                       const uint8_t *__restrict__ flags,
+                                                       const uint8_t *__restrict__ mask,
+                                                       const DomainDev *__restrict__ table,
+                                                       const uint2 *__restrict__ list, int rows) {
+    __shared__ typename St::T ring[kRing][8][kBlock];
+    const uint2 e = list[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+    const DomainDev d = table[e.x];
+    const int strip = (int)(e.y >> 16), r0 = (int)(e.y & 0xFFFFu) * rows;
+    pair_block<St, MODE>(ring, A + d.f_off, B + d.f_off, A2 + d.f_off, B2 + d.f_off, col0 + d.col0_off,
+                         flags + d.aux_off, mask + d.aux_off, d.g, strip, r0, min(r0 + rows, d.g.Y));
+}
+
+// C_{n+1} of every domain's column 0 for the outlets' wrapped pulls: one block per domain
+template <class St, int MODE>
+__global__ void k_batch_pair_col0(const typename St::T *A, const typename St::T *B, const uint8_t *__restrict__ flags,
+                                  const uint8_t *__restrict__ mask, const DomainDev *__restrict__ table,
+                                  typename St::T *__restrict__ col0) {
+    const DomainDev d = table[blockIdx.x];
+    for (int r = threadIdx.x; r < d.g.Y; r += blockDim.x)
+        pair_col0_row<St, MODE>(A + d.f_off, B + d.f_off, flags + d.aux_off, mask + d.aux_off, d.g, r,
+                                col0 + d.col0_off);
+}
+
 thread_local std::string g_batch_create_error;
 
 }  // namespace
@@ -122,6 +155,14 @@ struct lbm_batch {
     DomainDev *dtable = nullptr;
     uint2 *ilist = nullptr, *blist = nullptr, *alist = nullptr;
     uint32_t n_interior = 0, n_boundary = 0, n_all = 0;
+    // two-step variants: the output sets, every domain's column-0 buffer and
+    // the (domain, strip, row chunk) work list for `plist_rows`-row chunks
+    void *spare[2] = {nullptr, nullptr};
+    void *col0 = nullptr;
+    size_t col0_elems = 0;
+    uint2 *plist = nullptr;
+    uint32_t n_pair = 0;
+    int plist_rows = 0;
     int list_cpt = 0;  // tile width (cells per thread) the interior list was built for
     bool dirty = true;  // masks changed since the lists were built
     int cur = 0;
@@ -129,24 +170,30 @@ struct lbm_batch {
     uint64_t steps = 0;
     int variant = 0;
     bool use_graph = true;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};  // two fused steps starting from cur = 0 / 1
+    // fused steps starting from cur = 0 / 1 and, for the two-step variants, from
+    // either arrangement of the state and spare sets (each two-step launch
+    // trades them; a graph holds the pointers it was captured with)
+    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int pair_parity = 0;  // flips with every two-step launch
     int graph_variant = -1;
     int grid_blocks = 1024;
     std::string err;
     uint64_t cells = 0;
 
     void drop_graphs() {
-        for (auto &gx : graph)
-            if (gx) {
-                (void)hipGraphExecDestroy(gx);
-                gx = nullptr;
-            }
+        for (auto &row : graph)
+            for (auto &gx : row)
+                if (gx) {
+                    (void)hipGraphExecDestroy(gx);
+                    gx = nullptr;
+                }
         graph_variant = -1;
     }
 
     void release() {
         (void)hipSetDevice(dev);
         drop_graphs();
+        release_pair_buffers();
         for (void *p : {f[0], f[1], (void *)mask, (void *)flags, (void *)segs, (void *)lag, (void *)dense,
                         (void *)dtable, (void *)ilist, (void *)blist, (void *)alist})
             if (p) (void)hipFree(p);
@@ -158,6 +205,14 @@ struct lbm_batch {
     //   v        0  1  2  3  4  5
     //   cells    1  1  2  2  4  4      (f16s: 2 2 4 4 8 8)
     //   chunked  0  1  0  1  0  1
+    // variants 6, 7: two steps per launch (k_batch_pair) in chunks of 8 / 16
+    // rows; their one-step launches (the first and last steps of a call) run
+    // as the two-cells-per-thread variant (f32 2, binary16 0)
+    static constexpr int kPairFirst = 6;
+    static bool pair_variant(int v) { return v >= kPairFirst; }
+    int one_step_variant() const {
+        return pair_variant(variant) ? (precision == LBM_PRECISION_F32 ? 2 : 0) : variant;
+    }
     int cpt_of(int v) const {
         const int c = v / 2 == 0 ? 1 : (v / 2 == 1 ? 2 : 4);
         return precision == LBM_PRECISION_F32 ? c : 2 * c;
@@ -225,7 +280,7 @@ struct lbm_batch {
     }
 
     void ensure_lists() {
-        const int cpt = cpt_of(variant);
+        const int cpt = cpt_of(one_step_variant());
         if (dirty || list_cpt != cpt) rebuild_lists(cpt);
     }
 
@@ -249,8 +304,9 @@ struct lbm_batch {
     template <class St, int MODE, int OP>
     void pstep(const void *src, void *dst, float *mom) {
         constexpr bool f32 = sizeof(typename St::T) == 4;
-        const int chunked = variant & 1;
-        switch (variant / 2) {
+        const int v = one_step_variant();
+        const int chunked = v & 1;
+        switch (v / 2) {
         case 0: interior<St, MODE, OP, (f32 ? 1 : 2), true>(src, dst, mom, chunked); break;
         case 1: interior<St, MODE, OP, (f32 ? 2 : 4), true>(src, dst, mom, chunked); break;
         default: interior<St, MODE, OP, (f32 ? 4 : 8), true>(src, dst, mom, chunked); break;
@@ -280,6 +336,94 @@ struct lbm_batch {
         pform = false;
     }
 
+    // ---- two steps per launch ---------------------------------------------------------
+    size_t set_bytes() const { return f_elems * esize; }
+
+    void release_pair_buffers() {
+        drop_graphs();
+        for (void *&q : spare) {
+            if (q) (void)hipFree(q);
+            q = nullptr;
+        }
+        if (col0) (void)hipFree(col0);
+        col0 = nullptr;
+        if (plist) (void)hipFree(plist);
+        plist = nullptr;
+        n_pair = 0;
+        plist_rows = 0;
+    }
+
+    // allocated when a two-step variant is selected, so that stepping itself does not allocate
+    void ensure_pair_buffers() {
+        if (spare[0] && spare[1] && col0) return;
+        release_pair_buffers();
+        try {
+            spare[0] = dalloc<char>(set_bytes());
+            spare[1] = dalloc<char>(set_bytes());
+            col0 = dalloc<char>(std::max<size_t>(1, col0_elems) * esize);
+            // the rows' padding is never written by the two-step kernel: keep it defined
+            HIPCHK(hipMemsetAsync(spare[0], 0, set_bytes(), stream));
+            HIPCHK(hipMemsetAsync(spare[1], 0, set_bytes(), stream));
+        } catch (...) {
+            release_pair_buffers();
+            throw;
+        }
+    }
+
+    int pair_rows() const { return 8 << (variant - kPairFirst); }
+
+    void ensure_pair_list() {
+        const int rows = pair_rows();
+        if (plist && plist_rows == rows) return;
+        std::vector<uint2> pl;
+        for (size_t i = 0; i < table.size(); ++i) {
+            const Geom &g = table[i].g;
+            const int nstrips = (g.X + kBlock - 3) / (kBlock - 2), nchunks = (g.Y + rows - 1) / rows;
+            for (int c = 0; c < nchunks; ++c)
+                for (int sx = 0; sx < nstrips; ++sx)
+                    pl.push_back(make_uint2((uint32_t)i, ((uint32_t)sx << 16) | (uint32_t)c));
+        }
+        uint2 *d = dalloc<uint2>(pl.size());
+        try {
+            HIPCHK(hipMemcpyAsync(d, pl.data(), pl.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        if (plist) (void)hipFree(plist);
+        plist = d;
+        n_pair = (uint32_t)pl.size();
+        plist_rows = rows;
+        drop_graphs();
+    }
+
+    // C_n, C_{n-1} in f[cur], f[1 - cur] -> C_{n+2}, C_{n+1} (the latter only
+    // where the next step reads it); the spare sets take the old ones
+    template <class St, int MODE>
+    void launch_pair() {
+        using T = typename St::T;
+        const T *A = (const T *)f[cur], *B = (const T *)f[1 - cur];
+        T *A2 = (T *)spare[0], *B2 = (T *)spare[1];
+        if (MODE == LBM_BOUNDARY_REFERENCE) {
+            hipLaunchKernelGGL((k_batch_pair_col0<St, MODE>), dim3((unsigned)table.size()), dim3(256), 0, stream, A,
+                               B, flags, mask, dtable, (T *)col0);
+            HIPCHK(hipGetLastError());
+        }
+        hipLaunchKernelGGL((k_batch_pair<St, MODE>), dim3(n_pair), dim3(kBlock), 0, stream, A, B, A2, B2,
+                           (const T *)col0, flags, mask, dtable, plist, pair_rows());
+        HIPCHK(hipGetLastError());
+        spare[0] = f[cur];
+        spare[1] = f[1 - cur];
+        f[cur] = A2;
+        f[1 - cur] = B2;
+        pair_parity ^= 1;
+    }
+
+    void pair() {
+        dispatch([&]<class St, int MODE>() { launch_pair<St, MODE>(); });
+    }
+
     template <class F>
     void dispatch(F &&fn) {
         by_store(precision, [&]<class St>() {
@@ -292,19 +436,26 @@ struct lbm_batch {
         dispatch([&]<class St, int MODE>() { flush_t<St, MODE>(); });
     }
 
-    // two fused steps from cur = c, captured once and replayed
+    // two fused steps from cur = c (four for the two-step variants), captured
+    // once per arrangement of the sets and replayed
     hipGraphExec_t pair_graph(int c) {
         if (graph_variant != variant) drop_graphs();
-        if (!graph[c]) {
+        hipGraphExec_t &gx = graph[c][pair_parity];
+        if (!gx) {
             const int saved = cur;
             cur = c;
             hipGraph_t gr;
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             try {
-                dispatch([&]<class St, int MODE>() {
-                    do_step<St, MODE>(false);
-                    do_step<St, MODE>(false);
-                });
+                if (pair_variant(variant)) {
+                    pair();  // f[] and spare[] trade places ...
+                    pair();  // ... and trade back: the graph leaves them as it found them
+                } else {
+                    dispatch([&]<class St, int MODE>() {
+                        do_step<St, MODE>(false);
+                        do_step<St, MODE>(false);
+                    });
+                }
             } catch (...) {
                 hipGraph_t dummy;
                 (void)hipStreamEndCapture(stream, &dummy);
@@ -313,29 +464,42 @@ struct lbm_batch {
                 throw;
             }
             HIPCHK(hipStreamEndCapture(stream, &gr));
-            hipError_t e = hipGraphInstantiate(&graph[c], gr, nullptr, nullptr, 0);
+            hipError_t e = hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0);
             (void)hipGraphDestroy(gr);
             cur = saved;
             HIPCHK(e);
             graph_variant = variant;
         }
-        return graph[c];
+        return gx;
     }
 
     void step(uint32_t n) {
         if (n == 0) return;
         ensure_lists();
+        const bool two = pair_variant(variant);
+        if (two) {
+            ensure_pair_buffers();
+            ensure_pair_list();
+        }
         uint32_t left = n;
         if (!pform) {
             dispatch([&]<class St, int MODE>() { do_step<St, MODE>(left == 1); });
             --left;
         }
-        // all but the last step from the graph, two at a time
+        // all but the last step from the graph: two steps at a time, or four
+        // (two two-step launches) for the two-step variants
+        const uint32_t per = two ? 4 : 2;
         if (use_graph) {
-            while (left > 2) {
+            while (left > per) {
                 HIPCHK(hipGraphLaunch(pair_graph(cur), stream));
-                left -= 2;  // the graph leaves cur where it started
+                left -= per;  // the graph leaves cur (and the sets) where it started
             }
+        }
+        // two-step launches while at least one step is left for the closing
+        // one-step launch, which leaves both sets whole
+        while (two && left >= 3) {
+            pair();
+            left -= 2;
         }
         while (left > 0) {
             dispatch([&]<class St, int MODE>() { do_step<St, MODE>(left == 1); });
@@ -430,6 +594,8 @@ int lbm_batch_create(const lbm_domain *doms, uint32_t n, int32_t precision, int3
         d.seg_off = so;
         d.g = Geom{X, Y, pitch, pst, 9LL * pitch, (long long)pitch, doms[i].omega};
         d.g.sh = make_f16_shift(precision == LBM_PRECISION_F16E, doms[i].u_in);  // each domain's own free stream
+        d.col0_off = (long long)b->col0_elems;
+        b->col0_elems += 9 * (size_t)Y;
         b->table.push_back(d);
         fo += (9LL * pitch * Y + 255) / 256 * 256;
         ao += pst;
@@ -570,6 +736,9 @@ int lbm_batch_set_graph(lbm_batch *b, int enable) {
 int lbm_batch_set_variant(lbm_batch *b, int variant) {
     return bguard(b, [&] {
         bcheck(variant >= 0 && variant < LBM_BATCH_NUM_VARIANTS, "unknown variant");
+        HIPCHK(hipStreamSynchronize(b->stream));
+        if (lbm_batch::pair_variant(variant)) b->ensure_pair_buffers();  // may fail: then nothing changes
+        else b->release_pair_buffers();
         b->variant = variant;
     });
 }
@@ -588,6 +757,7 @@ int lbm_batch_autotune(lbm_batch *b, uint32_t steps, int *best, float *ms) {
         const int cur = b->cur;
         const bool pform = b->pform;
         const uint64_t nsteps = b->steps;
+        const int variant0 = b->variant;
         float t[LBM_BATCH_NUM_VARIANTS];
         int v_best = 0;
         try {
@@ -598,6 +768,18 @@ int lbm_batch_autotune(lbm_batch *b, uint32_t steps, int *best, float *ms) {
             HIPCHK(hipEventCreate(&e0));
             HIPCHK(hipEventCreate(&e1));
             for (int v = 0; v < LBM_BATCH_NUM_VARIANTS; ++v) {
+                if (lbm_batch::pair_variant(v)) {
+                    if (steps < 3) {  // a call this short never launches two steps at once
+                        t[v] = INFINITY;
+                        continue;
+                    }
+                    try {
+                        b->ensure_pair_buffers();
+                    } catch (...) {  // no memory for two more sets: not a candidate
+                        t[v] = INFINITY;
+                        continue;
+                    }
+                }
                 b->variant = v;
                 b->step(2);  // builds lists and graphs, leaves P-form
                 HIPCHK(hipEventRecord(e0, b->stream));
@@ -616,6 +798,10 @@ int lbm_batch_autotune(lbm_batch *b, uint32_t steps, int *best, float *ms) {
         } catch (...) {
             (void)hipStreamSynchronize(b->stream);
             (void)hipFree(snap);
+            b->cur = cur;
+            b->pform = pform;
+            b->steps = nsteps;
+            b->variant = variant0;
             throw;
         }
         (void)hipFree(snap);
@@ -623,6 +809,7 @@ int lbm_batch_autotune(lbm_batch *b, uint32_t steps, int *best, float *ms) {
         b->pform = pform;
         b->steps = nsteps;
         b->variant = v_best;
+        if (!lbm_batch::pair_variant(v_best)) b->release_pair_buffers();  // tried, not kept
         if (best) *best = v_best;
         if (ms)
             for (int v = 0; v < LBM_BATCH_NUM_VARIANTS; ++v) ms[v] = t[v] / steps;

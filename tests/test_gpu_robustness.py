@@ -217,7 +217,7 @@ def test_autotune_cache_selects_the_stored_variant(tmp_path):
     b = Batch(doms)
     assert b.autotune_cached(str(path), 4) == (bb, True)
     b.close()
-    assert any("/batch/4d/1Mcells/f32/ref/v6" in k for k in json.loads(path.read_text()))
+    assert any("/batch/4d/1Mcells/f32/ref/v8" in k for k in json.loads(path.read_text()))
     # the C++ driver: tuned and stored, then read
     cli = os.path.join(os.path.dirname(__file__), "..", "lbm_amd", "lbm_cli")
     cpath = tmp_path / "cli.json"
