@@ -893,10 +893,13 @@ ConvIn one_input(const uint16_t *a, int c) {
 constexpr int kTiles3[13][3] = {{1, 1, 1}, {2, 1, 1}, {4, 1, 1}, {1, 2, 1}, {2, 2, 1}, {1, 1, 2}, {2, 1, 2},
                                 {1, 2, 2}, {2, 2, 2}, {1, 1, 4}, {2, 1, 4}, {1, 2, 4}, {2, 2, 4}};
 
-// Register tile / block shape and K split of a convolution launch.  tile: index
-// into kTiles3 plus one (0: choose); split: input-chunk splits (0: choose).
+// Register tile / block shape, K split and kernel of a convolution launch.
+// tile: index into kTiles3 plus one (0: choose); split: input-chunk splits (0:
+// choose); kind: 0 k_conv3_lds (weights per wave from global memory), 1
+// k_conv3_lw (weights in LDS, 32-channel chunks), 2 k_conv3_lw with 64-channel
+// chunks, -1 choose.
 struct ConvTile {
-    int tile = 0, split = 0, kind = -1;  // kind: 0 weights per wave from global, 1 weights in LDS, -1 choose
+    int tile = 0, split = 0, kind = -1;
 };
 
 struct ConvPlan {
@@ -973,9 +976,11 @@ size_t conv_split_elems(int ks, long long P, int N, int H, int W, int Cin, int C
     return pl.split > 1 ? (size_t)pl.split * P * Cout : 0;
 }
 
-// Launch a convolution (also used for input gradients): 3x3 on k_conv3_lds,
-// 1x1 on k_conv1x1.  Only channels co < co_store are stored (all of them when
-// the K split is taken).
+// Launch a convolution (also used for input gradients): 3x3 on k_conv3_lds or
+// k_conv3_lw (conv3_plan), 1x1 on k_conv1x1.  Only channels co < co_store are
+// stored (all of them when the K split is taken).  pool_out: also the output's
+// 2 x 2 average pool; head: also a 1x1 head on the output (ConvHead); both in
+// the epilogue where the block shape allows, else as passes of their own.
 void conv_fwd(hipStream_t s, int ks, const ConvIn &in, const uint16_t *w, const float *bias, void *out, int N, int H,
               int W, int Cout, bool relu, bool out_f32, int co_store, float *ws, size_t ws_elems,
               ConvTile force = ConvTile{}, uint16_t *pool_out = nullptr, ConvHead head = ConvHead{}) {
