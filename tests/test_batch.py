@@ -127,6 +127,34 @@ def test_batch_domains_equal_single_grids(precision, variant):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 2])
+@pytest.mark.parametrize("variant", [0, 6, 7])
+def test_batch_zero_gradient_domains_equal_single_grids(precision, variant):
+    """Zero-gradient outlet (the outlet column's east pulls clamp instead of
+    wrapping): every domain equals a single grid of that mode bit for bit, for a
+    one-step variant and both two-step variants, after calls of 1, 37 and 2
+    steps."""
+    print(f"case: batch zero-gradient precision {precision} variant {variant}", flush=True)
+    from lbm_amd import BOUNDARY_ZERO_GRADIENT, LBM, Batch
+    doms, masks = _small_batch()
+    b = Batch(doms, precision=precision, boundary=BOUNDARY_ZERO_GRADIENT)
+    b.variant = variant
+    for i, m in enumerate(masks):
+        b.set_mask(i, m)
+    for n in (1, 37, 2):
+        b.step(n)
+    for i, ((nx, ny, om, u), m) in enumerate(zip(doms, masks)):
+        g = LBM(om, nx, ny, precision=precision, u_init=u, boundary=BOUNDARY_ZERO_GRADIENT)
+        g.mask = m
+        g.step(40)
+        assert np.array_equal(bits(b.read_f(i)), bits(g.read_f())), f"domain {i}"
+        for a, c in zip(b.moments(i), g.moments(lagged=True)):
+            assert np.array_equal(bits(a), bits(c)), f"moments of domain {i}"
+        g.close()
+    b.close()
+
+
+@pytest.mark.gpu
 def test_batch_graph_variants_and_mask_change():
     from lbm_amd import Batch
     doms, masks = _small_batch()

@@ -392,6 +392,29 @@ def test_every_kernel_variant_gives_the_same_bits(precision, variant):
         assert_state_equal(g, r, f"variant {variant}")
 
 
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_every_kernel_variant_zero_gradient_outlet(precision):
+    """Zero-gradient outlet: every step-kernel variant gives variant 0's bits
+    (variant 0 in this mode is pinned against the FP64 restatement above)."""
+    print(f"case: zero-gradient variants precision {precision}", flush=True)
+    from lbm_amd import BOUNDARY_ZERO_GRADIENT, LBM
+    X, Y = 203, 61
+    mask = circle_mask(X, Y, [(50, 30, 8), (140, 20, 5), (202, 40, 3)])
+    ref = None
+    for variant in range(10):
+        g = LBM(1.8, X, Y, precision=precision, boundary=BOUNDARY_ZERO_GRADIENT)
+        g.mask = mask
+        g.custom_speed(0.09)
+        g.variant = variant
+        g.step(120)
+        f = g.read_f()
+        g.close()
+        if ref is None:
+            ref = f
+        else:
+            assert same_bits(f, ref), f"variant {variant}"
+
+
 def both_sets(g):
     """Both parity sets of the saved state as (2, ny, 9, nx) raw elements (the
     rows' padding, which holds no state, cut off)."""
