@@ -39,7 +39,6 @@ struct DomainDev {
     long long f_off;     // element offset of the domain in each parity set
     long long aux_off;   // element offset in the mask / flags planes (and moment planes)
     long long seg_off;   // offset in the segment plane
-    long long col0_off;  // element offset of the domain's 9 x Y column-0 buffer (two-step variants)
     Geom g;
 };
 
@@ -111,7 +110,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_boundary(const typename St::T 
 template <class St, int MODE>
 __global__ __launch_bounds__(kBlock) void k_batch_pair(const typename St::T *A, const typename St::T *B,
                                                        typename St::T *__restrict__ A2, typename St::T *__restrict__ B2,
-                                                       const typename St::T *__restrict__ col0,
                                                        const uint8_t *__restrict__ flags,
                                                        const uint8_t *__restrict__ mask,
                                                        const DomainDev *__restrict__ table,
@@ -120,19 +118,19 @@ __global__ __launch_bounds__(kBlock) void k_batch_pair(const typename St::T *A, 
     const uint2 e = list[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
     const DomainDev d = table[e.x];
     const int strip = (int)(e.y >> 16), r0 = (int)(e.y & 0xFFFFu) * rows;
-    pair_block<St, MODE>(ring, A + d.f_off, B + d.f_off, A2 + d.f_off, B2 + d.f_off, col0 + d.col0_off,
-                         flags + d.aux_off, mask + d.aux_off, d.g, strip, r0, min(r0 + rows, d.g.Y));
+    pair_block_any<St, MODE>(ring, A + d.f_off, B + d.f_off, A2 + d.f_off, B2 + d.f_off, flags + d.aux_off,
+                             mask + d.aux_off, d.g, strip, r0, min(r0 + rows, d.g.Y));
 }
 
-// C_{n+1} of every domain's column 0 for the outlets' wrapped pulls: one block per domain
+// every domain's column 0 two steps on (pair_col0_row): one block per domain
 template <class St, int MODE>
 __global__ void k_batch_pair_col0(const typename St::T *A, const typename St::T *B, const uint8_t *__restrict__ flags,
                                   const uint8_t *__restrict__ mask, const DomainDev *__restrict__ table,
-                                  typename St::T *__restrict__ col0) {
+                                  typename St::T *__restrict__ A2, typename St::T *__restrict__ B2) {
     const DomainDev d = table[blockIdx.x];
     for (int r = threadIdx.x; r < d.g.Y; r += blockDim.x)
         pair_col0_row<St, MODE>(A + d.f_off, B + d.f_off, flags + d.aux_off, mask + d.aux_off, d.g, r,
-                                col0 + d.col0_off);
+                                A2 + d.f_off, B2 + d.f_off);
 }
 
 thread_local std::string g_batch_create_error;
@@ -146,7 +144,7 @@ struct lbm_batch {
     size_t esize = 4;
     std::vector<lbm_domain> doms;
     std::vector<DomainDev> table;
-    size_t f_elems = 0, aux_elems = 0, seg_bytes = 0;
+    size_t f_elems = 0, aux_elems = 0, seg_bytes = 0, tail_elems = 0;
     void *f[2] = {nullptr, nullptr};
     uint8_t *mask = nullptr, *flags = nullptr, *segs = nullptr;
     float *lag = nullptr;  // per domain 3 moment planes at 3 * aux_off
@@ -155,11 +153,9 @@ struct lbm_batch {
     DomainDev *dtable = nullptr;
     uint2 *ilist = nullptr, *blist = nullptr, *alist = nullptr;
     uint32_t n_interior = 0, n_boundary = 0, n_all = 0;
-    // two-step variants: the output sets, every domain's column-0 buffer and
-    // the (domain, strip, row chunk) work list for `plist_rows`-row chunks
+    // two-step variants: the output sets and the (domain, strip, row chunk)
+    // work list for `plist_rows`-row chunks
     void *spare[2] = {nullptr, nullptr};
-    void *col0 = nullptr;
-    size_t col0_elems = 0;
     uint2 *plist = nullptr;
     uint32_t n_pair = 0;
     int plist_rows = 0;
@@ -338,6 +334,9 @@ struct lbm_batch {
 
     // ---- two steps per launch ---------------------------------------------------------
     size_t set_bytes() const { return f_elems * esize; }
+    // a set's allocation: a row of the widest domain more, which the two-step
+    // kernel's wrapped outlet pulls in the last domain's last rows read and discard
+    size_t alloc_bytes() const { return (f_elems + tail_elems) * esize; }
 
     void release_pair_buffers() {
         drop_graphs();
@@ -345,8 +344,6 @@ struct lbm_batch {
             if (q) (void)hipFree(q);
             q = nullptr;
         }
-        if (col0) (void)hipFree(col0);
-        col0 = nullptr;
         if (plist) (void)hipFree(plist);
         plist = nullptr;
         n_pair = 0;
@@ -355,12 +352,11 @@ struct lbm_batch {
 
     // allocated when a two-step variant is selected, so that stepping itself does not allocate
     void ensure_pair_buffers() {
-        if (spare[0] && spare[1] && col0) return;
+        if (spare[0] && spare[1]) return;
         release_pair_buffers();
         try {
-            spare[0] = dalloc<char>(set_bytes());
-            spare[1] = dalloc<char>(set_bytes());
-            col0 = dalloc<char>(std::max<size_t>(1, col0_elems) * esize);
+            spare[0] = dalloc<char>(alloc_bytes());
+            spare[1] = dalloc<char>(alloc_bytes());
             // the rows' padding is never written by the two-step kernel: keep it defined
             HIPCHK(hipMemsetAsync(spare[0], 0, set_bytes(), stream));
             HIPCHK(hipMemsetAsync(spare[1], 0, set_bytes(), stream));
@@ -405,13 +401,11 @@ struct lbm_batch {
         using T = typename St::T;
         const T *A = (const T *)f[cur], *B = (const T *)f[1 - cur];
         T *A2 = (T *)spare[0], *B2 = (T *)spare[1];
-        if (MODE == LBM_BOUNDARY_REFERENCE) {
-            hipLaunchKernelGGL((k_batch_pair_col0<St, MODE>), dim3((unsigned)table.size()), dim3(256), 0, stream, A,
-                               B, flags, mask, dtable, (T *)col0);
-            HIPCHK(hipGetLastError());
-        }
+        hipLaunchKernelGGL((k_batch_pair_col0<St, MODE>), dim3((unsigned)table.size()), dim3(256), 0, stream, A, B,
+                           flags, mask, dtable, A2, B2);
+        HIPCHK(hipGetLastError());
         hipLaunchKernelGGL((k_batch_pair<St, MODE>), dim3(n_pair), dim3(kBlock), 0, stream, A, B, A2, B2,
-                           (const T *)col0, flags, mask, dtable, plist, pair_rows());
+                           flags, mask, dtable, plist, pair_rows());
         HIPCHK(hipGetLastError());
         spare[0] = f[cur];
         spare[1] = f[1 - cur];
@@ -594,21 +588,20 @@ int lbm_batch_create(const lbm_domain *doms, uint32_t n, int32_t precision, int3
         d.seg_off = so;
         d.g = Geom{X, Y, pitch, pst, 9LL * pitch, (long long)pitch, doms[i].omega};
         d.g.sh = make_f16_shift(precision == LBM_PRECISION_F16E, doms[i].u_in);  // each domain's own free stream
-        d.col0_off = (long long)b->col0_elems;
-        b->col0_elems += 9 * (size_t)Y;
         b->table.push_back(d);
         fo += (9LL * pitch * Y + 255) / 256 * 256;
         ao += pst;
         so += ((long long)(pitch / 64 + kSegSlack) * Y + 15) / 16 * 16;
         b->cells += (uint64_t)X * Y;
+        b->tail_elems = std::max(b->tail_elems, 9 * (size_t)pitch);
     }
     b->f_elems = fo;
     b->aux_elems = ao;
     b->seg_bytes = so;
     int rc = bguard(b, [&] {
         HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-        b->f[0] = dalloc<char>(b->f_elems * b->esize);
-        b->f[1] = dalloc<char>(b->f_elems * b->esize);
+        b->f[0] = dalloc<char>(b->alloc_bytes());
+        b->f[1] = dalloc<char>(b->alloc_bytes());
         b->mask = dalloc<uint8_t>(b->aux_elems);
         b->flags = dalloc<uint8_t>(b->aux_elems);
         b->segs = dalloc<uint8_t>(b->seg_bytes);

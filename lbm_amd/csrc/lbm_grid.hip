@@ -20,7 +20,8 @@
 //                      values elsewhere; buf[1-cur] := collide(S_n); swap.
 //   flush   (P -> S)   buf[1-cur] := S_n (streamed cells only); swap.
 //   pair    (P -> P -> P, variants 8 and 9) two fused steps in one pass, the
-//                      first one's result staged in LDS (k_pair below).
+//                      first one's result staged in LDS (k_pair below), after
+//                      column 0, which reads only itself (k_pair_col0).
 // Since the pull of step n uses the mask at the time the kernel runs, any mask
 // change first flushes a pending stream, so results are those of the reference
 // step order exactly.  The rest population is stored in both sets (the
@@ -81,23 +82,23 @@ __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, type
 
 template <class St, int MODE>
 __global__ void k_pair_col0(const typename St::T *A, const typename St::T *B, const uint8_t *__restrict__ flags,
-                            const uint8_t *__restrict__ mask, Geom g, typename St::T *__restrict__ col0) {
+                            const uint8_t *__restrict__ mask, Geom g, typename St::T *__restrict__ A2,
+                            typename St::T *__restrict__ B2) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= g.Y) return;
-    pair_col0_row<St, MODE>(A, B, flags, mask, g, r, col0);
+    pair_col0_row<St, MODE>(A, B, flags, mask, g, r, A2, B2);
 }
 
 template <class St, int MODE>
 __global__ __launch_bounds__(kBlock) void k_pair(const typename St::T *A, const typename St::T *B,
                                                  typename St::T *__restrict__ A2, typename St::T *__restrict__ B2,
-                                                 const typename St::T *__restrict__ col0,
                                                  const uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask,
                                                  Geom g, int nstrips, int rows, int nblocks, int remap) {
     __shared__ typename St::T ring[kRing][8][kBlock];
     const int L = remap ? xcd_remap(blockIdx.x, nblocks) : (int)blockIdx.x;
     const int chunk = L / nstrips;
     const int r0 = chunk * rows;
-    pair_block<St, MODE>(ring, A, B, A2, B2, col0, flags, mask, g, L - chunk * nstrips, r0, min(r0 + rows, g.Y));
+    pair_block_any<St, MODE>(ring, A, B, A2, B2, flags, mask, g, L - chunk * nstrips, r0, min(r0 + rows, g.Y));
 }
 
 // ---- surrogate support ---------------------------------------------------------
@@ -270,7 +271,6 @@ struct lbm_grid {
     }
     void *f[2] = {nullptr, nullptr};
     void *spare[2] = {nullptr, nullptr};  // output sets of the two-step kernel, allocated on first use
-    void *col0 = nullptr;                 // its outlet-wrap column (9 x ny elements)
     int cur = 0;
     bool pform = false;
     uint8_t *mask = nullptr, *flags = nullptr;
@@ -306,6 +306,9 @@ struct lbm_grid {
     size_t nvpart = 0;
 
     size_t set_bytes() const { return (size_t)geom.ys * Y * esize; }
+    // a set's allocation: one row more, which the two-step kernel's wrapped
+    // outlet pulls of the last rows read and discard
+    size_t alloc_bytes() const { return set_bytes() + (size_t)geom.ys * esize; }
     struct Shape {
         int blocks_per_row, nblocks;
     };
@@ -317,7 +320,7 @@ struct lbm_grid {
 
     void release() {
         (void)hipSetDevice(dev);
-        for (void *p : {f[0], f[1], spare[0], spare[1], col0, (void *)mask, (void *)flags,
+        for (void *p : {f[0], f[1], spare[0], spare[1], (void *)mask, (void *)flags,
                         (void *)lag, (void *)now, (void *)out, (void *)dense, (void *)partials, (void *)sums,
                         (void *)snap, (void *)umark, (void *)vpart})
             if (p) (void)hipFree(p);
@@ -325,7 +328,7 @@ struct lbm_grid {
         umark = nullptr;
         vpart = nullptr;
         f[0] = f[1] = nullptr;
-        spare[0] = spare[1] = col0 = nullptr;
+        spare[0] = spare[1] = nullptr;
         mask = flags = nullptr;
         lag = now = out = dense = nullptr;
         partials = sums = nullptr;
@@ -393,15 +396,14 @@ struct lbm_grid {
     static bool pair_variant(int v) { return v >= kPairFirst; }
     int pair_rows() const { return 8 << (variant - kPairFirst); }
 
-    // the two-step kernel's output sets and column buffer; allocated when a
-    // two-step variant is set, so that stepping itself does not allocate
+    // the two-step kernel's output sets; allocated when a two-step variant is
+    // set, so that stepping itself does not allocate
     void ensure_pair_buffers() {
-        if (spare[0] && spare[1] && col0) return;
+        if (spare[0] && spare[1]) return;
         release_pair_buffers();
         try {
-            spare[0] = dalloc<char>(set_bytes());
-            spare[1] = dalloc<char>(set_bytes());
-            col0 = dalloc<char>(9 * (size_t)Y * esize);
+            spare[0] = dalloc<char>(alloc_bytes());
+            spare[1] = dalloc<char>(alloc_bytes());
             // the rows' padding is never written by this kernel: keep it defined
             HIPCHK(hipMemsetAsync(spare[0], 0, set_bytes(), stream));
             HIPCHK(hipMemsetAsync(spare[1], 0, set_bytes(), stream));
@@ -417,16 +419,14 @@ struct lbm_grid {
         ensure_pair_buffers();
         const T *A = (const T *)f[cur], *B = (const T *)f[1 - cur];
         T *A2 = (T *)spare[0], *B2 = (T *)spare[1];
-        if (MODE == LBM_BOUNDARY_REFERENCE) {
-            hipLaunchKernelGGL((k_pair_col0<St, MODE>), dim3((Y + 255) / 256), dim3(256), 0, stream, A, B, flags,
-                               mask, geom, (T *)col0);
-            HIPCHK(hipGetLastError());
-        }
+        hipLaunchKernelGGL((k_pair_col0<St, MODE>), dim3((Y + 255) / 256), dim3(256), 0, stream, A, B, flags, mask,
+                           geom, A2, B2);
+        HIPCHK(hipGetLastError());
         const int rows = pair_rows();
         const int nstrips = ((int)X + kBlock - 3) / (kBlock - 2);
         const int nblocks = nstrips * (((int)Y + rows - 1) / rows);
         hipLaunchKernelGGL((k_pair<St, MODE>), dim3(nblocks), dim3(kBlock), 0, stream, A, B, A2, B2,
-                           (const T *)col0, flags, mask, geom, nstrips, rows, nblocks, 1);
+                           flags, mask, geom, nstrips, rows, nblocks, 1);
         HIPCHK(hipGetLastError());
         spare[0] = f[cur];
         spare[1] = f[1 - cur];
@@ -441,8 +441,6 @@ struct lbm_grid {
             if (p) (void)hipFree(p);
             p = nullptr;
         }
-        if (col0) (void)hipFree(col0);
-        col0 = nullptr;
     }
 
     // two P-form steps in one launch; the other set is then whole only on the
@@ -698,8 +696,8 @@ int lbm_create(const lbm_config *cfg, lbm_grid **out) {
     g->geom = Geom{(int)cfg->nx, (int)cfg->ny, pitch, pstride, 9LL * pitch, (long long)pitch, cfg->omega};
     int rc = guarded(g, [&] {
         HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
-        g->f[0] = dalloc<char>(g->set_bytes());
-        g->f[1] = dalloc<char>(g->set_bytes());
+        g->f[0] = dalloc<char>(g->alloc_bytes());
+        g->f[1] = dalloc<char>(g->alloc_bytes());
         g->mask = dalloc<uint8_t>(pstride);
         g->flags = dalloc<uint8_t>(pstride);
         g->lag = dalloc<float>(3 * (size_t)pstride);

@@ -11,6 +11,7 @@ namespace {
 // P-form -> P-form twice in one pass over HBM.  Inputs A = buf[cur] (C_n) and
 // B = buf[1-cur]; outputs A2 := C_{n+2} and B2 := C_{n+1}, in two other buffers
 // (a block's halo reads its neighbours' inputs, so nothing is updated in place).
+//
 // A block owns a strip of kBlock - 2 columns and `rows` rows; thread t works on
 // column x0 - 1 + t, so threads 0 and kBlock - 1 compute the strip's halo
 // columns.  Walking down the strip, phase 1 computes C_{n+1} = collide(S_{n+1})
@@ -18,23 +19,48 @@ namespace {
 // phase 2 pulls S_{n+2} of the current row from the ring and stores
 // collide(S_{n+2}) to A2.  C_{n+1} goes to B2 only where the next step reads
 // it (cells the stream does not write); a call therefore ends with one-step
-// launches, which leave both sets whole.  The outlet column's wrapped pulls read column 0, which
-// belongs to another strip; its C_{n+1} (column 0 is never streamed) comes
-// from k_pair_col0, launched first.  Same per-cell operations as k_cell, so
-// the same bits.
+// launches, which leave both sets whole.
+//
+// Column 0 is never streamed, so its two steps need no neighbour: the
+// column-0 pass (pair_col0_row, launched first) writes them into column 0 of
+// B2 (C_{n+1}) and A2 (C_{n+2}).
+//
+// In the strip kernel the lane of column 0 only relays the three east-moving
+// populations of C_{n+1} from B2 into the ring, for column 1's pulls; with the
+// reference outlet the lane of column X relays the west-moving populations of
+// column 0 one row down, which the outlet's pulls reach through the flattened
+// index.  So no lane takes a per-cell path for the grid's edges, and every
+// load of a row that does not depend on its flags is issued one row ahead (a
+// wave holding an edge cell used to wait on dependent loads in every row, and
+// its block with it at every barrier).  Same per-cell operations as k_cell,
+// so the same bits.
 constexpr int kRing = 4;
 
-// C_{n+1} of column 0, row r, into col0[k * Y + r] (the outlet's wrapped pulls
-// read it; column 0 belongs to another strip)
+// Column 0, row r: C_{n+1} = collide(S_{n+1}) into B2, C_{n+2} into A2.  An
+// unstreamed cell's S takes the moving populations of the other set (B for
+// step n+1, A for step n+2) and its own rest population.
 template <class St, int MODE>
 __device__ __forceinline__ void pair_col0_row(const typename St::T *A, const typename St::T *B,
                                               const uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask,
-                                              const Geom &g, int r, typename St::T *__restrict__ col0) {
+                                              const Geom &g, int r, typename St::T *__restrict__ A2,
+                                              typename St::T *__restrict__ B2) {
+    using T = typename St::T;
+    const long long ks = g.ks;
+    const size_t sown = (size_t)r * g.ys;
     float f[9];
-    gather_cell<St, MODE, true>(A, B, flags, mask, g, 0, r, (size_t)r * g.pitch, (size_t)r * g.ys, f);
+    gather_cell<St, MODE, true>(A, B, flags, mask, g, 0, r, (size_t)r * g.pitch, sown, f);
+    St::collide(f, g.omega);
+    T o[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        o[k] = St::store(f[k], k, g.sh);
+        B2[(size_t)k * ks + sown] = o[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f[k] = k == C ? St::load(o[C], C, g.sh) : St::load(A[(size_t)k * ks + sown], k, g.sh);
     St::collide(f, g.omega);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) col0[(size_t)k * g.Y + r] = St::store(f[k], k, g.sh);
+    for (int k = 0; k < 9; ++k) A2[(size_t)k * ks + sown] = St::store(f[k], k, g.sh);
 }
 
 // Barrier for LDS hand-off only: waits for this wave's LDS operations, not for
@@ -46,70 +72,99 @@ __device__ __forceinline__ void lds_barrier() {
 // One block's work of the two-step kernel: the strip of kBlock - 2 owner
 // columns from strip * (kBlock - 2), rows [r0, r1), with `ring` (C_{n+1} of
 // four rows: the eight moving populations; the rest population is only read
-// by its own cell, which keeps it in a register) in the block's LDS.
-template <class St, int MODE>
+// by its own cell, which keeps it in a register) in the block's LDS.  EDGE:
+// the strip holds column 0, X - 1 or X (pair_block_any picks the instance,
+// so that the other blocks run a loop without the relay and wrap code).
+template <class St, int MODE, bool EDGE>
 __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlock], const typename St::T *A,
                                            const typename St::T *B, typename St::T *__restrict__ A2,
-                                           typename St::T *__restrict__ B2, const typename St::T *__restrict__ col0,
-                                           const uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask,
-                                           const Geom &g, int strip, int r0, int r1) {
+                                           typename St::T *__restrict__ B2, const uint8_t *__restrict__ flags,
+                                           const uint8_t *__restrict__ mask, const Geom &g, int strip, int r0,
+                                           int r1) {
     using T = typename St::T;
+    constexpr bool wrap = MODE == LBM_BOUNDARY_REFERENCE;
     const long long ks = g.ks;
     const int t = (int)threadIdx.x;
     const int x = strip * (kBlock - 2) - 1 + t;
     const bool in_grid = x >= 0 && x < g.X;
-    const bool owner = t >= 1 && t <= kBlock - 2 && x < g.X;
-    // cells whose pulls leave the row (column 0, the outlet column) take the
-    // general per-cell path, as in the one-step kernel
-    const bool edge = x == 0 || x + 1 >= g.X;
-    // lane columns of the pulls, clamped into the row so every lane loads in
-    // bounds (lanes outside the grid, x = -1 or x >= nx, load and discard)
+    // column 0 is the column-0 pass's: its lane only relays C_{n+1} (as does
+    // the lane of column X with the reference outlet)
+    const bool owner = t >= 1 && t <= kBlock - 2 && x >= 1 && x < g.X;
+    const bool lane0 = EDGE && x == 0, laneX = EDGE && wrap && x == g.X, outlet = EDGE && x == g.X - 1;
+    // Byte offsets of a lane's pulls from the pulled row of A, by the sign of
+    // c_x: the row's columns x + 1, x, x - 1, clamped into the row so every
+    // lane loads in bounds (lanes outside the grid, x = -1 or x >= nx, load and
+    // discard; the zero-gradient outlet's clamped upstream is its own column).
+    // The reference outlet's west-moving pulls wrap to column 0 of the next
+    // row (the flattened index): one row on, column 0; in the last rows that is
+    // the row past the set, which every set's allocation holds (those pulls are
+    // flagged as bounce-back).  The relay lanes read column 0 of B2, which the
+    // column-0 pass has written: column 0's lane the pulled row itself, column
+    // X's lane (reference outlet) the row after it.  The offsets are
+    // loop-invariant, so every lane issues the same loads.
+    const long long es = (long long)sizeof(T), row_b = g.ys * es;
+    const long long dB = (long long)((uintptr_t)B2 - (uintptr_t)A);
+    long long cw = (long long)min(max(x + 1, 0), g.X - 1) * es;
+    long long cc = (long long)min(max(x, 0), g.X - 1) * es;
+    long long ce = (long long)min(max(x - 1, 0), g.X - 1) * es;
+    if (lane0) ce = dB;
+    if (outlet && wrap) cw = row_b;
+    if (laneX) cw = dB + row_b;
+    // row 0's south-west pull from row -1 wraps to row 0 itself, the clamped
+    // row (only the first rows are loaded before the loop: `first`)
+    const long long cw_row0 = (outlet && wrap) || laneX ? cw - row_b : cw;
     const unsigned xc = (unsigned)min(max(x, 0), g.X - 1);
-    const unsigned xw = (unsigned)min(max(x - 1, 0), g.X - 1);
-    const unsigned xe = (unsigned)min(max(x + 1, 0), g.pitch - 1);
 
     // phase 1, part 1: the nine pulls of row r, issued at clamped addresses
     // (the rows are uniform: clamped once per block); patched in part 2
-    auto load1 = [&](int r, T v[9], uint32_t &fl) {
+    auto load1 = [&](int r, T v[9], uint32_t &fl, bool first) {
         if (r < 0 || r >= g.Y) return;
         fl = flags[(size_t)r * g.pitch + xc];
-        const T *up = A + (size_t)min(r + 1, g.Y - 1) * g.ys;  // cy = -1 pulls from r + 1
-        const T *mid = A + (size_t)r * g.ys;
-        const T *dn = A + (size_t)max(r - 1, 0) * g.ys;        // cy = +1 pulls from r - 1
+        const char *up = (const char *)(A + (size_t)min(r + 1, g.Y - 1) * g.ys);  // cy = -1 pulls from r + 1
+        const char *mid = (const char *)(A + (size_t)r * g.ys);
+        const char *dn = (const char *)(A + (size_t)max(r - 1, 0) * g.ys);        // cy = +1 pulls from r - 1
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            const T *row = cy(k) < 0 ? up : (cy(k) > 0 ? dn : mid);
-            const unsigned col = cx(k) < 0 ? xe : (cx(k) > 0 ? xw : xc);
-            v[k] = row[(size_t)k * ks + col];
+            const char *row = cy(k) < 0 ? up : (cy(k) > 0 ? dn : mid);
+            long long col = cx(k) < 0 ? cw : (cx(k) > 0 ? ce : cc);
+            if (first && cx(k) < 0 && cy(k) > 0 && r < 1) col = cw_row0;
+            v[k] = *(const T *)(row + k * ks * es + col);
         }
     };
-    // phase 1, part 2: S_{n+1} of row r (bounce-back, unstreamed cells, edges),
-    // C_{n+1} = collide into the ring; returns the rest population
+    // phase 1, part 2: S_{n+1} of row r (bounce-back, unstreamed cells),
+    // C_{n+1} = collide into the ring; returns the rest population.  A relay
+    // lane puts what it loaded, C_{n+1} of column 0 at the pulled row (column
+    // X's: the row after it), into the ring slot of the pulled row, where the
+    // neighbour's second-step pull finds it; the slots are those of the
+    // unclamped rows, and the clamped ones are never read.
     auto finish1 = [&](int r, const T v[9], uint32_t fl) -> T {
-        if (!in_grid || r < 0 || r >= g.Y) return T{};
+        if (r < 0 || r >= g.Y) return T{};
+        if (lane0 || laneX) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                if ((cx(k) > 0 && lane0) || (cx(k) < 0 && laneX)) ring[(r - cy(k)) & (kRing - 1)][bit(k)][t] = v[k];
+            return T{};
+        }
+        if (!in_grid) return T{};
         const size_t own = (size_t)r * g.pitch + x;
         const size_t sown = (size_t)r * g.ys + x;
         float f[9];
         bool streamed = true;
-        if (edge) {
-            streamed = gather_cell<St, MODE, true>(A, B, flags, mask, g, x, r, own, sown, f);
-        } else {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) f[k] = St::load(v[k], k, g.sh);
-            if (fl == kSlowCell) {
-                streamed = mask[own] != 1 && r < g.Y - 1;
+        for (int k = 0; k < 9; ++k) f[k] = St::load(v[k], k, g.sh);
+        if (fl == kSlowCell) {
+            streamed = mask[own] != 1 && r < g.Y - 1;
 #pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    if (k == C) continue;
-                    f[k] = St::load_b(streamed ? A[(size_t)opp(k) * ks + sown] : B[(size_t)k * ks + sown], k,
-                                      streamed, g.sh);
-                }
-            } else if (fl != 0) {
+            for (int k = 0; k < 9; ++k) {
+                if (k == C) continue;
+                f[k] = St::load_b(streamed ? A[(size_t)opp(k) * ks + sown] : B[(size_t)k * ks + sown], k,
+                                  streamed, g.sh);
+            }
+        } else if (fl != 0) {
 #pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    if (k == C) continue;
-                    if ((fl >> bit(k)) & 1u) f[k] = St::load(A[(size_t)opp(k) * ks + sown], opp(k), g.sh);
-                }
+            for (int k = 0; k < 9; ++k) {
+                if (k == C) continue;
+                if ((fl >> bit(k)) & 1u) f[k] = St::load(A[(size_t)opp(k) * ks + sown], opp(k), g.sh);
             }
         }
         St::collide(f, g.omega);
@@ -142,25 +197,23 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
         f[C] = St::load(rest, C, g.sh);
         if (fl == kSlowCell) {
             const size_t sown = (size_t)y * g.ys + x;
-            const bool streamed = mask[(size_t)y * g.pitch + x] != 1 && x != 0 && y < g.Y - 1;
+            const bool streamed = mask[(size_t)y * g.pitch + x] != 1 && y < g.Y - 1;
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 if (k == C) continue;
                 f[k] = St::load_b(streamed ? ring[me][bit(opp(k))][t] : A[(size_t)k * ks + sown], k, streamed, g.sh);
             }
-        } else if (fl != 0 || x == g.X - 1) {
+        } else if (fl != 0 || (!wrap && outlet)) {
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 if (k == C) continue;
                 if ((fl >> bit(k)) & 1u) {
                     f[k] = St::load(ring[me][bit(opp(k))][t], opp(k), g.sh);
-                } else if (x - cx(k) >= g.X) {
-                    // the outlet's pull past the last column: the flattened
-                    // index wraps to column 0 of the next row, or clamps
-                    if constexpr (MODE == LBM_BOUNDARY_REFERENCE)
-                        f[k] = St::load(col0[(size_t)k * g.Y + (y - cy(k)) + 1], k, g.sh);
-                    else
-                        f[k] = St::load(ring[(y - cy(k)) & (kRing - 1)][bit(k)][t], k, g.sh);
+                } else if (!wrap && x - cx(k) >= g.X) {
+                    // the zero-gradient outlet's pull past the last column
+                    // clamps to the cell's own column (the reference outlet's
+                    // wraps: column X's relay lane holds it)
+                    f[k] = St::load(ring[(y - cy(k)) & (kRing - 1)][bit(k)][t], k, g.sh);
                 }
             }
         }
@@ -174,18 +227,18 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
     // finished into the ring, barrier, row y's second step
     T v[9];
     uint32_t fl = 0;
-    load1(r0 - 1, v, fl);
+    load1(r0 - 1, v, fl, true);
     finish1(r0 - 1, v, fl);
-    load1(r0, v, fl);
+    load1(r0, v, fl, true);
     T rest = finish1(r0, v, fl);
     uint32_t fl_y = fl;
-    load1(r0 + 1, v, fl);
+    load1(r0 + 1, v, fl, true);
     for (int y = r0; y < r1; ++y) {
         T cv[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) cv[k] = v[k];
         const uint32_t cfl = fl;
-        if (y + 2 <= r1) load1(y + 2, v, fl);
+        if (y + 2 <= r1) load1(y + 2, v, fl, false);
         const T next = finish1(y + 1, cv, cfl);
         lds_barrier();
         phase2(y, rest, fl_y);
@@ -194,5 +247,16 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
     }
 }
 
+template <class St, int MODE>
+__device__ __forceinline__ void pair_block_any(typename St::T (&ring)[kRing][8][kBlock], const typename St::T *A,
+                                               const typename St::T *B, typename St::T *__restrict__ A2,
+                                               typename St::T *__restrict__ B2,
+                                               const uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask,
+                                               const Geom &g, int strip, int r0, int r1) {
+    if (strip == 0 || strip * (kBlock - 2) + kBlock - 2 >= g.X - 1)
+        pair_block<St, MODE, true>(ring, A, B, A2, B2, flags, mask, g, strip, r0, r1);
+    else
+        pair_block<St, MODE, false>(ring, A, B, A2, B2, flags, mask, g, strip, r0, r1);
+}
 
 }  // namespace

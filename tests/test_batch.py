@@ -155,6 +155,32 @@ def test_batch_zero_gradient_domains_equal_single_grids(precision, variant):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 2])
+@pytest.mark.parametrize("variant", [6, 7])
+def test_batch_open_rows_equal_single_grids(precision, variant):
+    """Domains without wall rows (their outlets' pulls wrap between rows at the
+    top and bottom, the last domain's into the padding past the set) equal
+    single grids bit for bit in the two-step variants."""
+    print(f"case: batch open rows precision {precision} variant {variant}", flush=True)
+    from lbm_amd import LBM, Batch, gen_shapes
+    doms, _ = _small_batch()
+    masks = [gen_shapes(nx, ny, 900 + i, 1, 6, walls=False) for i, (nx, ny, _, _) in enumerate(doms)]
+    b = Batch(doms, precision=precision)
+    b.variant = variant
+    for i, m in enumerate(masks):
+        b.set_mask(i, m)
+    for n in (1, 37, 2):
+        b.step(n)
+    for i, ((nx, ny, om, u), m) in enumerate(zip(doms, masks)):
+        g = LBM(om, nx, ny, precision=precision, u_init=u)
+        g.mask = m
+        g.step(40)
+        assert np.array_equal(bits(b.read_f(i)), bits(g.read_f())), f"domain {i}"
+        g.close()
+    b.close()
+
+
+@pytest.mark.gpu
 def test_batch_graph_variants_and_mask_change():
     from lbm_amd import Batch
     doms, masks = _small_batch()

@@ -512,6 +512,48 @@ def test_two_step_variants_small_and_boundary_sizes(X, Y):
                 b.close()
 
 
+@pytest.mark.parametrize("X,Y,fill", [(254, 19, 0.25), (255, 17, 0.25), (509, 33, 0.3), (64, 9, 0.2),
+                                      (1018, 21, 0.25), (300, 12, 0.0)])
+def test_two_step_variants_open_rows_and_random_obstacles(X, Y, fill):
+    """No wall rows: the first and last rows are fluid, so the outlet's pulls wrap
+    between rows at the grid's top and bottom (row 0's south-west pull wraps to
+    column 0 of row 0, the last rows' to the row past the set, flagged), and
+    random solid cells touch column 0, the outlet column and each other,
+    enclosing some fluid cells.  Both sets after calls of odd and even length
+    equal the one-step kernel's, and (f32, reference outlet) the oracle's."""
+    print(f"case: two-step open rows {X}x{Y} fill {fill}", flush=True)
+    from lbm_amd import LBM
+    rng = np.random.default_rng(X * 1000 + Y)
+    mask = (rng.random((Y, X)) < fill).astype(np.uint8)
+    for precision in (0, 1, 2):
+        for boundary in (0, 1):
+            for variant in (8, 9):
+                grids = []
+                for v in (0, variant):
+                    g = LBM(1.7, X, Y, precision=precision, boundary=boundary)
+                    g.mask = mask
+                    g.custom_speed(0.08)
+                    g.variant = v
+                    grids.append(g)
+                a, b = grids
+                for n in (7, 4):
+                    a.step(n)
+                    b.step(n)
+                    sa, sb = both_sets(a), both_sets(b)
+                    if not np.array_equal(sa, sb):
+                        d = np.argwhere(sa != sb)
+                        pytest.fail(f"{(precision, boundary, variant)} after {n}: {len(d)} differ, "
+                                    f"first (set,y,k,x)={tuple(d[0])}")
+                if precision == 0 and boundary == 0:
+                    r = oracle.RefCPU(X, Y, 1.7)
+                    r.set_mask(mask.astype(np.uint32))
+                    r.reset_equilibrium(1.0, 0.08, 0.0)
+                    r.step(11)
+                    assert_state_equal(b, r, f"two-step variant {variant}, open rows")
+                a.close()
+                b.close()
+
+
 def test_switching_between_one_and_two_step_variants():
     """Leaving a two-step variant frees its extra sets; switching back and forth
     between calls keeps the state equal to the one-step kernel's."""
