@@ -111,26 +111,55 @@ template <class St, int MODE>
 __global__ __launch_bounds__(kBlock) void k_batch_pair(const typename St::T *A, const typename St::T *B,
                                                        typename St::T *__restrict__ A2, typename St::T *__restrict__ B2,
                                                        const uint8_t *__restrict__ flags,
-                                                       const uint8_t *__restrict__ mask,
                                                        const DomainDev *__restrict__ table,
                                                        const uint2 *__restrict__ list, int rows) {
     __shared__ typename St::T ring[kRing][8][kBlock];
     const uint2 e = list[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
     const DomainDev d = table[e.x];
     const int strip = (int)(e.y >> 16), r0 = (int)(e.y & 0xFFFFu) * rows;
-    pair_block_any<St, MODE>(ring, A + d.f_off, B + d.f_off, A2 + d.f_off, B2 + d.f_off, flags + d.aux_off,
-                             mask + d.aux_off, d.g, strip, r0, min(r0 + rows, d.g.Y));
+    pair_block_any<St, MODE>(ring, A + d.f_off, B + d.f_off, A2 + d.f_off, B2 + d.f_off, flags + d.aux_off, d.g,
+                             strip, r0, min(r0 + rows, d.g.Y));
 }
 
-// every domain's column 0 two steps on (pair_col0_row): one block per domain
-template <class St, int MODE>
-__global__ void k_batch_pair_col0(const typename St::T *A, const typename St::T *B, const uint8_t *__restrict__ flags,
-                                  const uint8_t *__restrict__ mask, const DomainDev *__restrict__ table,
-                                  typename St::T *__restrict__ A2, typename St::T *__restrict__ B2) {
-    const DomainDev d = table[blockIdx.x];
-    for (int r = threadIdx.x; r < d.g.Y; r += blockDim.x)
-        pair_col0_row<St, MODE>(A + d.f_off, B + d.f_off, flags + d.aux_off, mask + d.aux_off, d.g, r,
-                                A2 + d.f_off, B2 + d.f_off);
+// the fixed-cell list of every domain (pair.h): one block per entry
+// (domain, row) of the row table; entries (column | streamed << 31,
+// row-table index)
+__global__ __launch_bounds__(kFixedBlock) void k_batch_fixed_count(const uint8_t *__restrict__ flags,
+                                                                   const DomainDev *__restrict__ table,
+                                                                   const uint2 *__restrict__ rowtab,
+                                                                   uint32_t *__restrict__ counts) {
+    const uint2 e = rowtab[blockIdx.x];
+    const DomainDev d = table[e.x];
+    fixed_row_count(flags + d.aux_off + (size_t)e.y * d.g.pitch, d.g.X, counts + blockIdx.x);
+}
+
+__global__ __launch_bounds__(kFixedBlock) void k_batch_fixed_emit(const uint8_t *__restrict__ flags,
+                                                                  const uint8_t *__restrict__ mask,
+                                                                  const DomainDev *__restrict__ table,
+                                                                  const uint2 *__restrict__ rowtab,
+                                                                  const uint32_t *__restrict__ offsets,
+                                                                  uint2 *__restrict__ list) {
+    const uint2 e = rowtab[blockIdx.x];
+    const DomainDev d = table[e.x];
+    const size_t r = d.aux_off + (size_t)e.y * d.g.pitch;
+    fixed_row_emit(flags + r, mask + r, d.g.X, (int)e.y, d.g.Y, blockIdx.x, list + offsets[blockIdx.x]);
+}
+
+// the fixed-cell pass of every domain: one thread per list entry
+template <class St>
+__global__ __launch_bounds__(256) void k_batch_pair_fixed(const typename St::T *A, const typename St::T *B,
+                                                          const DomainDev *__restrict__ table,
+                                                          const uint2 *__restrict__ rowtab,
+                                                          const uint2 *__restrict__ list, uint32_t n,
+                                                          typename St::T *__restrict__ A2,
+                                                          typename St::T *__restrict__ B2) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint2 e = list[i];
+    const uint2 r = rowtab[e.y];
+    const DomainDev d = table[r.x];
+    fixed_cell<St>(A + d.f_off, B + d.f_off, d.g, (int)(e.x & 0x7FFFFFFFu), (int)r.y, (e.x >> 31) != 0, A2 + d.f_off,
+                   B2 + d.f_off);
 }
 
 thread_local std::string g_batch_create_error;
@@ -158,6 +187,10 @@ struct lbm_batch {
     void *spare[2] = {nullptr, nullptr};
     uint2 *plist = nullptr;
     uint32_t n_pair = 0;
+    uint2 *fixed = nullptr, *rowtab = nullptr;  // fixed-cell list (column | streamed << 31, row-table index); (domain, row)
+    uint32_t *fixed_rows = nullptr;             // the list's offset per row-table entry
+    uint32_t n_fixed = 0, n_rows = 0;
+    bool fixed_dirty = true;
     int plist_rows = 0;
     int list_cpt = 0;  // tile width (cells per thread) the interior list was built for
     bool dirty = true;  // masks changed since the lists were built
@@ -219,6 +252,7 @@ struct lbm_batch {
     // ---- lists -----------------------------------------------------------------
     void rebuild_lists(int cpt) {
         // bounce-back and segment bytes on the device, lists on the host
+        fixed_dirty = true;
         for (size_t i = 0; i < table.size(); ++i) {
             const DomainDev &d = table[i];
             const dim3 gf((d.g.X + 255) / 256, d.g.Y);
@@ -348,22 +382,64 @@ struct lbm_batch {
         plist = nullptr;
         n_pair = 0;
         plist_rows = 0;
+        for (void *q : {(void *)fixed, (void *)rowtab, (void *)fixed_rows})
+            if (q) (void)hipFree(q);
+        fixed = rowtab = nullptr;
+        fixed_rows = nullptr;
+        n_fixed = 0;
     }
 
-    // allocated when a two-step variant is selected, so that stepping itself does not allocate
+    // allocated when a two-step variant is selected, so that stepping itself
+    // does not allocate: the output sets, and the fixed-cell list (room for
+    // every cell) with its row table (domain, row) and per-row offsets
     void ensure_pair_buffers() {
-        if (spare[0] && spare[1]) return;
+        if (spare[0] && spare[1] && fixed) return;
         release_pair_buffers();
         try {
             spare[0] = dalloc<char>(alloc_bytes());
             spare[1] = dalloc<char>(alloc_bytes());
+            std::vector<uint2> rt;
+            for (size_t i = 0; i < table.size(); ++i)
+                for (int y = 0; y < table[i].g.Y; ++y) rt.push_back(make_uint2((uint32_t)i, (uint32_t)y));
+            n_rows = (uint32_t)rt.size();
+            fixed = dalloc<uint2>((size_t)cells);
+            rowtab = dalloc<uint2>(rt.size());
+            fixed_rows = dalloc<uint32_t>(rt.size());
             // the rows' padding is never written by the two-step kernel: keep it defined
             HIPCHK(hipMemsetAsync(spare[0], 0, set_bytes(), stream));
             HIPCHK(hipMemsetAsync(spare[1], 0, set_bytes(), stream));
+            HIPCHK(hipMemcpyAsync(rowtab, rt.data(), rt.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            fixed_dirty = true;
         } catch (...) {
             release_pair_buffers();
             throw;
         }
+    }
+
+    // the fixed-cell list from the current flags; waits for the stream.  Drops
+    // the graphs, whose fixed-cell launches hold the old count.
+    void build_fixed_list() {
+        hipLaunchKernelGGL(k_batch_fixed_count, dim3(n_rows), dim3(kFixedBlock), 0, stream, flags, dtable, rowtab,
+                           fixed_rows);
+        HIPCHK(hipGetLastError());
+        std::vector<uint32_t> off(n_rows);
+        HIPCHK(hipMemcpyAsync(off.data(), fixed_rows, off.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        uint32_t total = 0;
+        for (uint32_t &o : off) {
+            const uint32_t c = o;
+            o = total;
+            total += c;
+        }
+        HIPCHK(hipMemcpyAsync(fixed_rows, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_batch_fixed_emit, dim3(n_rows), dim3(kFixedBlock), 0, stream, flags, mask, dtable,
+                           rowtab, fixed_rows, fixed);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));  // `off` is read by the copy above
+        n_fixed = total;
+        fixed_dirty = false;
+        drop_graphs();
     }
 
     int pair_rows() const { return 8 << (variant - kPairFirst); }
@@ -401,11 +477,13 @@ struct lbm_batch {
         using T = typename St::T;
         const T *A = (const T *)f[cur], *B = (const T *)f[1 - cur];
         T *A2 = (T *)spare[0], *B2 = (T *)spare[1];
-        hipLaunchKernelGGL((k_batch_pair_col0<St, MODE>), dim3((unsigned)table.size()), dim3(256), 0, stream, A, B,
-                           flags, mask, dtable, A2, B2);
-        HIPCHK(hipGetLastError());
+        if (n_fixed) {
+            hipLaunchKernelGGL((k_batch_pair_fixed<St>), dim3((n_fixed + 255) / 256), dim3(256), 0, stream, A, B,
+                               dtable, rowtab, fixed, n_fixed, A2, B2);
+            HIPCHK(hipGetLastError());
+        }
         hipLaunchKernelGGL((k_batch_pair<St, MODE>), dim3(n_pair), dim3(kBlock), 0, stream, A, B, A2, B2,
-                           flags, mask, dtable, plist, pair_rows());
+                           flags, dtable, plist, pair_rows());
         HIPCHK(hipGetLastError());
         spare[0] = f[cur];
         spare[1] = f[1 - cur];
@@ -474,6 +552,7 @@ struct lbm_batch {
         if (two) {
             ensure_pair_buffers();
             ensure_pair_list();
+            if (fixed_dirty) build_fixed_list();
         }
         uint32_t left = n;
         if (!pform) {

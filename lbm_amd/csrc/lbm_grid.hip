@@ -21,7 +21,7 @@
 //   flush   (P -> S)   buf[1-cur] := S_n (streamed cells only); swap.
 //   pair    (P -> P -> P, variants 8 and 9) two fused steps in one pass, the
 //                      first one's result staged in LDS (k_pair below), after
-//                      column 0, which reads only itself (k_pair_col0).
+//                      the cells that read only themselves (k_pair_fixed).
 // Since the pull of step n uses the mask at the time the kernel runs, any mask
 // change first flushes a pending stream, so results are those of the reference
 // step order exactly.  The rest population is stored in both sets (the
@@ -78,27 +78,44 @@ __global__ __launch_bounds__(kBlock) void k_cell(const typename St::T *src, type
 }
 
 // ---- two steps per launch -------------------------------------------------------
-// (the per-block work is pair_block / pair_col0_row in pair.h)
+// (the per-block work is pair_block / fixed_cell in pair.h)
 
-template <class St, int MODE>
-__global__ void k_pair_col0(const typename St::T *A, const typename St::T *B, const uint8_t *__restrict__ flags,
-                            const uint8_t *__restrict__ mask, Geom g, typename St::T *__restrict__ A2,
-                            typename St::T *__restrict__ B2) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= g.Y) return;
-    pair_col0_row<St, MODE>(A, B, flags, mask, g, r, A2, B2);
+// the fixed-cell list: kSlowCell cells per row (one block per row), then the
+// (column, row) entries at each row's offset
+__global__ __launch_bounds__(kFixedBlock) void k_fixed_count(const uint8_t *__restrict__ flags, Geom g,
+                                                             uint32_t *__restrict__ counts) {
+    fixed_row_count(flags + (size_t)blockIdx.x * g.pitch, g.X, counts + blockIdx.x);
+}
+
+__global__ __launch_bounds__(kFixedBlock) void k_fixed_emit(const uint8_t *__restrict__ flags,
+                                                            const uint8_t *__restrict__ mask, Geom g,
+                                                            const uint32_t *__restrict__ offsets,
+                                                            uint2 *__restrict__ list) {
+    const size_t r = (size_t)blockIdx.x * g.pitch;
+    fixed_row_emit(flags + r, mask + r, g.X, (int)blockIdx.x, g.Y, blockIdx.x, list + offsets[blockIdx.x]);
+}
+
+// the fixed-cell pass: one thread per list entry (column | streamed << 31, row)
+template <class St>
+__global__ __launch_bounds__(256) void k_pair_fixed(const typename St::T *A, const typename St::T *B, Geom g,
+                                                    const uint2 *__restrict__ list, uint32_t n,
+                                                    typename St::T *__restrict__ A2, typename St::T *__restrict__ B2) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint2 e = list[i];
+    fixed_cell<St>(A, B, g, (int)(e.x & 0x7FFFFFFFu), (int)e.y, (e.x >> 31) != 0, A2, B2);
 }
 
 template <class St, int MODE>
 __global__ __launch_bounds__(kBlock) void k_pair(const typename St::T *A, const typename St::T *B,
                                                  typename St::T *__restrict__ A2, typename St::T *__restrict__ B2,
-                                                 const uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask,
-                                                 Geom g, int nstrips, int rows, int nblocks, int remap) {
+                                                 const uint8_t *__restrict__ flags, Geom g, int nstrips, int rows,
+                                                 int nblocks, int remap) {
     __shared__ typename St::T ring[kRing][8][kBlock];
     const int L = remap ? xcd_remap(blockIdx.x, nblocks) : (int)blockIdx.x;
     const int chunk = L / nstrips;
     const int r0 = chunk * rows;
-    pair_block_any<St, MODE>(ring, A, B, A2, B2, flags, mask, g, L - chunk * nstrips, r0, min(r0 + rows, g.Y));
+    pair_block_any<St, MODE>(ring, A, B, A2, B2, flags, g, L - chunk * nstrips, r0, min(r0 + rows, g.Y));
 }
 
 // ---- surrogate support ---------------------------------------------------------
@@ -271,6 +288,9 @@ struct lbm_grid {
     }
     void *f[2] = {nullptr, nullptr};
     void *spare[2] = {nullptr, nullptr};  // output sets of the two-step kernel, allocated on first use
+    uint2 *fixed = nullptr;               // its fixed-cell list: (column | streamed << 31, row) per kSlowCell cell
+    uint32_t *fixed_rows = nullptr;       // the list's offset per row
+    uint32_t n_fixed = 0;
     int cur = 0;
     bool pform = false;
     uint8_t *mask = nullptr, *flags = nullptr;
@@ -320,7 +340,7 @@ struct lbm_grid {
 
     void release() {
         (void)hipSetDevice(dev);
-        for (void *p : {f[0], f[1], spare[0], spare[1], (void *)mask, (void *)flags,
+        for (void *p : {f[0], f[1], spare[0], spare[1], (void *)fixed, (void *)fixed_rows, (void *)mask, (void *)flags,
                         (void *)lag, (void *)now, (void *)out, (void *)dense, (void *)partials, (void *)sums,
                         (void *)snap, (void *)umark, (void *)vpart})
             if (p) (void)hipFree(p);
@@ -329,6 +349,8 @@ struct lbm_grid {
         vpart = nullptr;
         f[0] = f[1] = nullptr;
         spare[0] = spare[1] = nullptr;
+        fixed = nullptr;
+        fixed_rows = nullptr;
         mask = flags = nullptr;
         lag = now = out = dense = nullptr;
         partials = sums = nullptr;
@@ -396,21 +418,46 @@ struct lbm_grid {
     static bool pair_variant(int v) { return v >= kPairFirst; }
     int pair_rows() const { return 8 << (variant - kPairFirst); }
 
-    // the two-step kernel's output sets; allocated when a two-step variant is
-    // set, so that stepping itself does not allocate
+    // the two-step kernel's output sets and fixed-cell list (room for every
+    // cell); allocated when a two-step variant is set, so that stepping itself
+    // does not allocate
     void ensure_pair_buffers() {
-        if (spare[0] && spare[1]) return;
+        if (spare[0] && spare[1] && fixed) return;
         release_pair_buffers();
         try {
             spare[0] = dalloc<char>(alloc_bytes());
             spare[1] = dalloc<char>(alloc_bytes());
+            fixed = dalloc<uint2>((size_t)X * Y);
+            fixed_rows = dalloc<uint32_t>((size_t)Y);
             // the rows' padding is never written by this kernel: keep it defined
             HIPCHK(hipMemsetAsync(spare[0], 0, set_bytes(), stream));
             HIPCHK(hipMemsetAsync(spare[1], 0, set_bytes(), stream));
+            build_fixed_list();
         } catch (...) {
             release_pair_buffers();
             throw;
         }
+    }
+
+    // the fixed-cell list from the current flags (after every mask change
+    // while the two-step buffers exist); waits for the stream
+    void build_fixed_list() {
+        hipLaunchKernelGGL(k_fixed_count, dim3(Y), dim3(kFixedBlock), 0, stream, flags, geom, fixed_rows);
+        HIPCHK(hipGetLastError());
+        std::vector<uint32_t> off(Y);
+        HIPCHK(hipMemcpyAsync(off.data(), fixed_rows, Y * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        uint32_t total = 0;
+        for (uint32_t &o : off) {
+            const uint32_t c = o;
+            o = total;
+            total += c;
+        }
+        HIPCHK(hipMemcpyAsync(fixed_rows, off.data(), Y * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_fixed_emit, dim3(Y), dim3(kFixedBlock), 0, stream, flags, mask, geom, fixed_rows, fixed);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));  // `off` is read by the copy above
+        n_fixed = total;
     }
 
     template <class St, int MODE>
@@ -419,14 +466,16 @@ struct lbm_grid {
         ensure_pair_buffers();
         const T *A = (const T *)f[cur], *B = (const T *)f[1 - cur];
         T *A2 = (T *)spare[0], *B2 = (T *)spare[1];
-        hipLaunchKernelGGL((k_pair_col0<St, MODE>), dim3((Y + 255) / 256), dim3(256), 0, stream, A, B, flags, mask,
-                           geom, A2, B2);
-        HIPCHK(hipGetLastError());
+        if (n_fixed) {
+            hipLaunchKernelGGL((k_pair_fixed<St>), dim3((n_fixed + 255) / 256), dim3(256), 0, stream, A, B, geom,
+                               fixed, n_fixed, A2, B2);
+            HIPCHK(hipGetLastError());
+        }
         const int rows = pair_rows();
         const int nstrips = ((int)X + kBlock - 3) / (kBlock - 2);
         const int nblocks = nstrips * (((int)Y + rows - 1) / rows);
         hipLaunchKernelGGL((k_pair<St, MODE>), dim3(nblocks), dim3(kBlock), 0, stream, A, B, A2, B2,
-                           flags, mask, geom, nstrips, rows, nblocks, 1);
+                           flags, geom, nstrips, rows, nblocks, 1);
         HIPCHK(hipGetLastError());
         spare[0] = f[cur];
         spare[1] = f[1 - cur];
@@ -434,13 +483,18 @@ struct lbm_grid {
         f[1 - cur] = B2;
     }
 
-    // the two-step kernel's output sets and column buffer (the state itself
+    // the two-step kernel's output sets and fixed-cell list (the state itself
     // is always in f[0], f[1] between calls)
     void release_pair_buffers() {
         for (void *&p : spare) {
             if (p) (void)hipFree(p);
             p = nullptr;
         }
+        if (fixed) (void)hipFree(fixed);
+        if (fixed_rows) (void)hipFree(fixed_rows);
+        fixed = nullptr;
+        fixed_rows = nullptr;
+        n_fixed = 0;
     }
 
     // two P-form steps in one launch; the other set is then whole only on the
@@ -470,6 +524,7 @@ struct lbm_grid {
             hipLaunchKernelGGL(k_build_flags<LBM_BOUNDARY_ZERO_GRADIENT>, grid, dim3(256), 0, stream, mask, flags,
                                geom);
         HIPCHK(hipGetLastError());
+        if (fixed) build_fixed_list();
     }
 
     void fill_both(const float v[9]) {

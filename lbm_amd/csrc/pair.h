@@ -12,55 +12,148 @@ namespace {
 // B = buf[1-cur]; outputs A2 := C_{n+2} and B2 := C_{n+1}, in two other buffers
 // (a block's halo reads its neighbours' inputs, so nothing is updated in place).
 //
+// Cells with the flag byte kSlowCell -- solid cells, column 0, the last row
+// (none of them streamed) and fluid cells whose eight upstream cells are all
+// solid (every population bounced) -- read only themselves over both steps.
+// The fixed-cell pass (fixed_cell over a list of them, launched first) steps
+// them into A2 and B2 and the strip kernel skips them: nothing pulls from them
+// but column 1 from column 0 and the row above from the last row, which the
+// strip kernel relays or recomputes.
+//
 // A block owns a strip of kBlock - 2 columns and `rows` rows; thread t works on
 // column x0 - 1 + t, so threads 0 and kBlock - 1 compute the strip's halo
 // columns.  Walking down the strip, phase 1 computes C_{n+1} = collide(S_{n+1})
 // of the next row, including the halo, into a ring of four rows in LDS, and
 // phase 2 pulls S_{n+2} of the current row from the ring and stores
 // collide(S_{n+2}) to A2.  C_{n+1} goes to B2 only where the next step reads
-// it (cells the stream does not write); a call therefore ends with one-step
-// launches, which leave both sets whole.
-//
-// Column 0 is never streamed, so its two steps need no neighbour: the
-// column-0 pass (pair_col0_row, launched first) writes them into column 0 of
-// B2 (C_{n+1}) and A2 (C_{n+2}).
+// it (cells the stream does not write: the fixed-cell pass's); a call
+// therefore ends with one-step launches, which leave both sets whole.
 //
 // In the strip kernel the lane of column 0 only relays the three east-moving
 // populations of C_{n+1} from B2 into the ring, for column 1's pulls; with the
 // reference outlet the lane of column X relays the west-moving populations of
 // column 0 one row down, which the outlet's pulls reach through the flattened
-// index.  So no lane takes a per-cell path for the grid's edges, and every
-// load of a row that does not depend on its flags is issued one row ahead (a
-// wave holding an edge cell used to wait on dependent loads in every row, and
-// its block with it at every barrier).  Same per-cell operations as k_cell,
-// so the same bits.
+// index.  So no lane takes a per-cell path for the grid's edges or its solid
+// cells, and the only global loads not issued one row ahead are bounce-back
+// reads (a wave holding an edge or solid cell used to wait on dependent loads
+// in every row, and its block with it at every barrier).  Same per-cell
+// operations as k_cell, so the same bits.
 constexpr int kRing = 4;
 
-// Column 0, row r: C_{n+1} = collide(S_{n+1}) into B2, C_{n+2} into A2.  An
-// unstreamed cell's S takes the moving populations of the other set (B for
-// step n+1, A for step n+2) and its own rest population.
-template <class St, int MODE>
-__device__ __forceinline__ void pair_col0_row(const typename St::T *A, const typename St::T *B,
-                                              const uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask,
-                                              const Geom &g, int r, typename St::T *__restrict__ A2,
-                                              typename St::T *__restrict__ B2) {
+// A kSlowCell cell (x, y) two steps on: C_{n+1} into B2 (where the cell is
+// not streamed: the next step reads it there), C_{n+2} into A2.  A cell that
+// is not streamed takes the other set's moving populations (B for step n+1, A
+// for step n+2) and keeps its rest population; a streamed one (every upstream
+// cell solid) bounces every population back from its own cell.  `streamed`
+// comes with the list entry, so every load is issued at once.
+template <class St>
+__device__ __forceinline__ void fixed_cell(const typename St::T *A, const typename St::T *B, const Geom &g, int x,
+                                           int y, bool streamed, typename St::T *__restrict__ A2,
+                                           typename St::T *__restrict__ B2) {
     using T = typename St::T;
     const long long ks = g.ks;
-    const size_t sown = (size_t)r * g.ys;
+    const size_t sown = (size_t)y * g.ys + x;
+    T a[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = A[(size_t)k * ks + sown];
     float f[9];
-    gather_cell<St, MODE, true>(A, B, flags, mask, g, 0, r, (size_t)r * g.pitch, sown, f);
+    if (streamed) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            if (k != C) f[k] = St::load_b(a[opp(k)], k, true, g.sh);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            if (k != C) f[k] = St::load(B[(size_t)k * ks + sown], k, g.sh);
+    }
+    f[C] = St::load(a[C], C, g.sh);
     St::collide(f, g.omega);
     T o[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         o[k] = St::store(f[k], k, g.sh);
-        B2[(size_t)k * ks + sown] = o[k];
+        if (!streamed) B2[(size_t)k * ks + sown] = o[k];
     }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) f[k] = k == C ? St::load(o[C], C, g.sh) : St::load(A[(size_t)k * ks + sown], k, g.sh);
+    for (int k = 0; k < 9; ++k) {
+        if (k == C) f[k] = St::load(o[C], C, g.sh);
+        else f[k] = streamed ? St::load_b(o[opp(k)], k, true, g.sh) : St::load(a[k], k, g.sh);
+    }
     St::collide(f, g.omega);
 #pragma unroll
     for (int k = 0; k < 9; ++k) A2[(size_t)k * ks + sown] = St::store(f[k], k, g.sh);
+}
+
+// ---- the fixed-cell list ------------------------------------------------------
+//
+// The fixed-cell pass runs one thread per entry of a list of the kSlowCell
+// cells, (column and whether it is streamed, row) in row order, built on the
+// device after each mask change: a count per row, an exclusive scan of the counts on the host, and
+// an ordered emit per row.  One block of kFixedBlock threads per row; each
+// thread looks at 16 flag bytes (one aligned 16-byte word: the flag plane's
+// rows are padded to a multiple of 64 bytes) per pass over the row.
+constexpr int kFixedBlock = 256;
+
+// bit i set: column x0 + i < X is a kSlowCell cell
+__device__ __forceinline__ uint32_t slow_bits16(const uint8_t *__restrict__ row, int x0, int X) {
+    if (x0 >= X) return 0;
+    const uint4 w = *(const uint4 *)(row + x0);
+    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (((ws[j] >> (8 * b)) & 0xFFu) == kSlowCell) m |= 1u << (4 * j + b);
+    return X - x0 < 16 ? m & ((1u << (X - x0)) - 1u) : m;
+}
+
+// exclusive prefix of c over the block's threads (in `sc`, kFixedBlock
+// entries); *total gets the sum
+__device__ __forceinline__ uint32_t block_prefix(uint32_t c, uint32_t *sc, uint32_t *total) {
+    const int t = (int)threadIdx.x;
+    sc[t] = c;
+    __syncthreads();
+    for (int off = 1; off < kFixedBlock; off <<= 1) {
+        const uint32_t v = t >= off ? sc[t - off] : 0u;
+        __syncthreads();
+        sc[t] += v;
+        __syncthreads();
+    }
+    const uint32_t incl = sc[t];
+    *total = sc[kFixedBlock - 1];
+    __syncthreads();
+    return incl - c;
+}
+
+// the number of kSlowCell cells of one row of X columns into *count
+__device__ __forceinline__ void fixed_row_count(const uint8_t *__restrict__ row, int X, uint32_t *count) {
+    __shared__ uint32_t sc[kFixedBlock];
+    uint32_t c = 0;
+    for (int x0 = (int)threadIdx.x * 16; x0 < X; x0 += kFixedBlock * 16) c += __popc(slow_bits16(row, x0, X));
+    uint32_t total;
+    block_prefix(c, sc, &total);
+    if (threadIdx.x == 0) *count = total;
+}
+
+// the kSlowCell cells of row y (of Y, X columns; `row` and `mrow` its flag and
+// mask bytes) as (column | streamed << 31, tag) into out[0 ..), in column order
+__device__ __forceinline__ void fixed_row_emit(const uint8_t *__restrict__ row, const uint8_t *__restrict__ mrow,
+                                               int X, int y, int Y, uint32_t tag, uint2 *__restrict__ out) {
+    __shared__ uint32_t sc[kFixedBlock];
+    uint32_t base = 0;
+    for (int x00 = 0; x00 < X; x00 += kFixedBlock * 16) {
+        const int x0 = x00 + (int)threadIdx.x * 16;
+        uint32_t m = slow_bits16(row, x0, X), total;
+        uint32_t at = base + block_prefix(__popc(m), sc, &total);
+        while (m) {
+            const int x = x0 + __builtin_ctz(m);
+            m &= m - 1u;
+            const bool streamed = mrow[x] != 1 && x != 0 && y < Y - 1;
+            out[at++] = make_uint2((uint32_t)x | (streamed ? 0x80000000u : 0u), tag);
+        }
+        base += total;
+    }
 }
 
 // Barrier for LDS hand-off only: waits for this wave's LDS operations, not for
@@ -79,8 +172,7 @@ template <class St, int MODE, bool EDGE>
 __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlock], const typename St::T *A,
                                            const typename St::T *B, typename St::T *__restrict__ A2,
                                            typename St::T *__restrict__ B2, const uint8_t *__restrict__ flags,
-                                           const uint8_t *__restrict__ mask, const Geom &g, int strip, int r0,
-                                           int r1) {
+                                           const Geom &g, int strip, int r0, int r1) {
     using T = typename St::T;
     constexpr bool wrap = MODE == LBM_BOUNDARY_REFERENCE;
     const long long ks = g.ks;
@@ -131,7 +223,7 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
             v[k] = *(const T *)(row + k * ks * es + col);
         }
     };
-    // phase 1, part 2: S_{n+1} of row r (bounce-back, unstreamed cells),
+    // phase 1, part 2: S_{n+1} of row r (bounce-back; the last row),
     // C_{n+1} = collide into the ring; returns the rest population.  A relay
     // lane puts what it loaded, C_{n+1} of column 0 at the pulled row (column
     // X's: the row after it), into the ring slot of the pulled row, where the
@@ -145,21 +237,18 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
                 if ((cx(k) > 0 && lane0) || (cx(k) < 0 && laneX)) ring[(r - cy(k)) & (kRing - 1)][bit(k)][t] = v[k];
             return T{};
         }
-        if (!in_grid) return T{};
-        const size_t own = (size_t)r * g.pitch + x;
+        // fixed cells are the fixed-cell pass's, and nothing pulls from them
+        // but the row above from the last row
+        if (!in_grid || (fl == kSlowCell && r != g.Y - 1)) return T{};
         const size_t sown = (size_t)r * g.ys + x;
         float f[9];
-        bool streamed = true;
 #pragma unroll
         for (int k = 0; k < 9; ++k) f[k] = St::load(v[k], k, g.sh);
-        if (fl == kSlowCell) {
-            streamed = mask[own] != 1 && r < g.Y - 1;
+        if (r == g.Y - 1) {
+            // the last row is not streamed: the other set's moving populations
 #pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                if (k == C) continue;
-                f[k] = St::load_b(streamed ? A[(size_t)opp(k) * ks + sown] : B[(size_t)k * ks + sown], k,
-                                  streamed, g.sh);
-            }
+            for (int k = 0; k < 9; ++k)
+                if (k != C) f[k] = St::load(B[(size_t)k * ks + sown], k, g.sh);
         } else if (fl != 0) {
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
@@ -168,17 +257,11 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
             }
         }
         St::collide(f, g.omega);
-        const bool store = owner && r >= r0 && r < r1 && !streamed;
         T o[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             o[k] = St::store(f[k], k, g.sh);
             if (k != C) ring[r & (kRing - 1)][bit(k)][t] = o[k];
-        }
-        if (store) {
-            T *dst = B2 + (size_t)r * g.ys;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) dst[(size_t)k * ks + x] = o[k];
         }
         return o[C];
     };
@@ -186,7 +269,7 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
     // phase 2: S_{n+2} of row y from the ring, collide, store to A2; fl is
     // the row's flag byte, read by phase 1
     auto phase2 = [&](int y, T rest, uint32_t fl) {
-        if (!owner) return;
+        if (!owner || fl == kSlowCell) return;  // fixed cells: the fixed-cell pass's
         const int me = y & (kRing - 1);
         float f[9];
 #pragma unroll
@@ -195,15 +278,7 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
             f[k] = St::load(ring[(y - cy(k)) & (kRing - 1)][bit(k)][t - cx(k)], k, g.sh);
         }
         f[C] = St::load(rest, C, g.sh);
-        if (fl == kSlowCell) {
-            const size_t sown = (size_t)y * g.ys + x;
-            const bool streamed = mask[(size_t)y * g.pitch + x] != 1 && y < g.Y - 1;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                if (k == C) continue;
-                f[k] = St::load_b(streamed ? ring[me][bit(opp(k))][t] : A[(size_t)k * ks + sown], k, streamed, g.sh);
-            }
-        } else if (fl != 0 || (!wrap && outlet)) {
+        if (fl != 0 || (!wrap && outlet)) {
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 if (k == C) continue;
@@ -251,12 +326,12 @@ template <class St, int MODE>
 __device__ __forceinline__ void pair_block_any(typename St::T (&ring)[kRing][8][kBlock], const typename St::T *A,
                                                const typename St::T *B, typename St::T *__restrict__ A2,
                                                typename St::T *__restrict__ B2,
-                                               const uint8_t *__restrict__ flags, const uint8_t *__restrict__ mask,
-                                               const Geom &g, int strip, int r0, int r1) {
+                                               const uint8_t *__restrict__ flags, const Geom &g, int strip,
+                                               int r0, int r1) {
     if (strip == 0 || strip * (kBlock - 2) + kBlock - 2 >= g.X - 1)
-        pair_block<St, MODE, true>(ring, A, B, A2, B2, flags, mask, g, strip, r0, r1);
+        pair_block<St, MODE, true>(ring, A, B, A2, B2, flags, g, strip, r0, r1);
     else
-        pair_block<St, MODE, false>(ring, A, B, A2, B2, flags, mask, g, strip, r0, r1);
+        pair_block<St, MODE, false>(ring, A, B, A2, B2, flags, g, strip, r0, r1);
 }
 
 }  // namespace
