@@ -105,16 +105,31 @@ __global__ __launch_bounds__(kBlock) void k_batch_boundary(const typename St::T 
 
 // Two steps per launch for every domain (the single grid's two-step scheme,
 // pair.h): one block per work item (domain, 254-column strip, row chunk) of
-// the pair list, blocks of one XCD on one contiguous run of it.  A, B: the
-// current sets; A2, B2: the spare sets that receive C_{n+2} and C_{n+1}.
+// the pair list, blocks of one XCD on one contiguous run of it, then the
+// fixed-cell pass's blocks.  A, B: the current sets; A2, B2: the spare sets
+// that receive C_{n+2} and C_{n+1}.
 template <class St, int MODE>
 __global__ __launch_bounds__(kBlock) void k_batch_pair(const typename St::T *A, const typename St::T *B,
                                                        typename St::T *__restrict__ A2, typename St::T *__restrict__ B2,
                                                        const uint8_t *__restrict__ flags,
                                                        const DomainDev *__restrict__ table,
-                                                       const uint2 *__restrict__ list, int rows) {
+                                                       const uint2 *__restrict__ list, int n_items, int rows,
+                                                       const uint2 *__restrict__ fixed,
+                                                       const uint2 *__restrict__ rowtab, uint32_t n_fixed) {
     __shared__ typename St::T ring[kRing][8][kBlock];
-    const uint2 e = list[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+    if ((int)blockIdx.x >= n_items) {
+        // the fixed-cell pass: one thread per list entry (column | streamed << 31, row-table index)
+        const uint32_t i = (blockIdx.x - n_items) * kBlock + threadIdx.x;
+        if (i < n_fixed) {
+            const uint2 f = fixed[i];
+            const uint2 r = rowtab[f.y];
+            const DomainDev d = table[r.x];
+            fixed_cell<St>(A + d.f_off, B + d.f_off, d.g, (int)(f.x & 0x7FFFFFFFu), (int)r.y, (f.x >> 31) != 0,
+                           A2 + d.f_off, B2 + d.f_off);
+        }
+        return;
+    }
+    const uint2 e = list[xcd_remap((int)blockIdx.x, n_items)];
     const DomainDev d = table[e.x];
     const int strip = (int)(e.y >> 16), r0 = (int)(e.y & 0xFFFFu) * rows;
     pair_block_any<St, MODE>(ring, A + d.f_off, B + d.f_off, A2 + d.f_off, B2 + d.f_off, flags + d.aux_off, d.g,
@@ -143,23 +158,6 @@ __global__ __launch_bounds__(kFixedBlock) void k_batch_fixed_emit(const uint8_t 
     const DomainDev d = table[e.x];
     const size_t r = d.aux_off + (size_t)e.y * d.g.pitch;
     fixed_row_emit(flags + r, mask + r, d.g.X, (int)e.y, d.g.Y, blockIdx.x, list + offsets[blockIdx.x]);
-}
-
-// the fixed-cell pass of every domain: one thread per list entry
-template <class St>
-__global__ __launch_bounds__(256) void k_batch_pair_fixed(const typename St::T *A, const typename St::T *B,
-                                                          const DomainDev *__restrict__ table,
-                                                          const uint2 *__restrict__ rowtab,
-                                                          const uint2 *__restrict__ list, uint32_t n,
-                                                          typename St::T *__restrict__ A2,
-                                                          typename St::T *__restrict__ B2) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint2 e = list[i];
-    const uint2 r = rowtab[e.y];
-    const DomainDev d = table[r.x];
-    fixed_cell<St>(A + d.f_off, B + d.f_off, d.g, (int)(e.x & 0x7FFFFFFFu), (int)r.y, (e.x >> 31) != 0, A2 + d.f_off,
-                   B2 + d.f_off);
 }
 
 thread_local std::string g_batch_create_error;
@@ -368,8 +366,8 @@ struct lbm_batch {
 
     // ---- two steps per launch ---------------------------------------------------------
     size_t set_bytes() const { return f_elems * esize; }
-    // a set's allocation: a row of the widest domain more, which the two-step
-    // kernel's wrapped outlet pulls in the last domain's last rows read and discard
+    // a set's allocation: two rows of the widest domain more, which the
+    // two-step kernel's loads in the last domain's last rows read and discard
     size_t alloc_bytes() const { return (f_elems + tail_elems) * esize; }
 
     void release_pair_buffers() {
@@ -477,13 +475,9 @@ struct lbm_batch {
         using T = typename St::T;
         const T *A = (const T *)f[cur], *B = (const T *)f[1 - cur];
         T *A2 = (T *)spare[0], *B2 = (T *)spare[1];
-        if (n_fixed) {
-            hipLaunchKernelGGL((k_batch_pair_fixed<St>), dim3((n_fixed + 255) / 256), dim3(256), 0, stream, A, B,
-                               dtable, rowtab, fixed, n_fixed, A2, B2);
-            HIPCHK(hipGetLastError());
-        }
-        hipLaunchKernelGGL((k_batch_pair<St, MODE>), dim3(n_pair), dim3(kBlock), 0, stream, A, B, A2, B2,
-                           flags, dtable, plist, pair_rows());
+        const unsigned nfix = (n_fixed + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL((k_batch_pair<St, MODE>), dim3(n_pair + nfix), dim3(kBlock), 0, stream, A, B, A2, B2,
+                           flags, dtable, plist, (int)n_pair, pair_rows(), fixed, rowtab, n_fixed);
         HIPCHK(hipGetLastError());
         spare[0] = f[cur];
         spare[1] = f[1 - cur];
@@ -672,7 +666,7 @@ int lbm_batch_create(const lbm_domain *doms, uint32_t n, int32_t precision, int3
         ao += pst;
         so += ((long long)(pitch / 64 + kSegSlack) * Y + 15) / 16 * 16;
         b->cells += (uint64_t)X * Y;
-        b->tail_elems = std::max(b->tail_elems, 9 * (size_t)pitch);
+        b->tail_elems = std::max(b->tail_elems, 18 * (size_t)pitch);
     }
     b->f_elems = fo;
     b->aux_elems = ao;

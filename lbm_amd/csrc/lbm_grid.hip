@@ -20,8 +20,8 @@
 //                      values elsewhere; buf[1-cur] := collide(S_n); swap.
 //   flush   (P -> S)   buf[1-cur] := S_n (streamed cells only); swap.
 //   pair    (P -> P -> P, variants 8 and 9) two fused steps in one pass, the
-//                      first one's result staged in LDS (k_pair below), after
-//                      the cells that read only themselves (k_pair_fixed).
+//                      first one's result staged in LDS (k_pair below); its
+//                      last blocks step the cells that read only themselves.
 // Since the pull of step n uses the mask at the time the kernel runs, any mask
 // change first flushes a pending stream, so results are those of the reference
 // step order exactly.  The rest population is stored in both sets (the
@@ -95,23 +95,22 @@ __global__ __launch_bounds__(kFixedBlock) void k_fixed_emit(const uint8_t *__res
     fixed_row_emit(flags + r, mask + r, g.X, (int)blockIdx.x, g.Y, blockIdx.x, list + offsets[blockIdx.x]);
 }
 
-// the fixed-cell pass: one thread per list entry (column | streamed << 31, row)
-template <class St>
-__global__ __launch_bounds__(256) void k_pair_fixed(const typename St::T *A, const typename St::T *B, Geom g,
-                                                    const uint2 *__restrict__ list, uint32_t n,
-                                                    typename St::T *__restrict__ A2, typename St::T *__restrict__ B2) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint2 e = list[i];
-    fixed_cell<St>(A, B, g, (int)(e.x & 0x7FFFFFFFu), (int)e.y, (e.x >> 31) != 0, A2, B2);
-}
-
 template <class St, int MODE>
 __global__ __launch_bounds__(kBlock) void k_pair(const typename St::T *A, const typename St::T *B,
                                                  typename St::T *__restrict__ A2, typename St::T *__restrict__ B2,
                                                  const uint8_t *__restrict__ flags, Geom g, int nstrips, int rows,
-                                                 int nblocks, int remap) {
+                                                 int nblocks, int remap, const uint2 *__restrict__ fixed,
+                                                 uint32_t n_fixed) {
     __shared__ typename St::T ring[kRing][8][kBlock];
+    if ((int)blockIdx.x >= nblocks) {
+        // the fixed-cell pass: one thread per list entry (column | streamed << 31, row)
+        const uint32_t i = (blockIdx.x - nblocks) * kBlock + threadIdx.x;
+        if (i < n_fixed) {
+            const uint2 e = fixed[i];
+            fixed_cell<St>(A, B, g, (int)(e.x & 0x7FFFFFFFu), (int)e.y, (e.x >> 31) != 0, A2, B2);
+        }
+        return;
+    }
     const int L = remap ? xcd_remap(blockIdx.x, nblocks) : (int)blockIdx.x;
     const int chunk = L / nstrips;
     const int r0 = chunk * rows;
@@ -326,9 +325,9 @@ struct lbm_grid {
     size_t nvpart = 0;
 
     size_t set_bytes() const { return (size_t)geom.ys * Y * esize; }
-    // a set's allocation: one row more, which the two-step kernel's wrapped
-    // outlet pulls of the last rows read and discard
-    size_t alloc_bytes() const { return set_bytes() + (size_t)geom.ys * esize; }
+    // a set's allocation: two rows more, which the two-step kernel's loads of
+    // the last rows (wrapped outlet pulls, the relay lanes) read and discard
+    size_t alloc_bytes() const { return set_bytes() + 2 * (size_t)geom.ys * esize; }
     struct Shape {
         int blocks_per_row, nblocks;
     };
@@ -466,16 +465,12 @@ struct lbm_grid {
         ensure_pair_buffers();
         const T *A = (const T *)f[cur], *B = (const T *)f[1 - cur];
         T *A2 = (T *)spare[0], *B2 = (T *)spare[1];
-        if (n_fixed) {
-            hipLaunchKernelGGL((k_pair_fixed<St>), dim3((n_fixed + 255) / 256), dim3(256), 0, stream, A, B, geom,
-                               fixed, n_fixed, A2, B2);
-            HIPCHK(hipGetLastError());
-        }
         const int rows = pair_rows();
         const int nstrips = ((int)X + kBlock - 3) / (kBlock - 2);
         const int nblocks = nstrips * (((int)Y + rows - 1) / rows);
-        hipLaunchKernelGGL((k_pair<St, MODE>), dim3(nblocks), dim3(kBlock), 0, stream, A, B, A2, B2,
-                           flags, geom, nstrips, rows, nblocks, 1);
+        const unsigned nfix = (n_fixed + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL((k_pair<St, MODE>), dim3(nblocks + nfix), dim3(kBlock), 0, stream, A, B, A2, B2, flags,
+                           geom, nstrips, rows, nblocks, 1, fixed, n_fixed);
         HIPCHK(hipGetLastError());
         spare[0] = f[cur];
         spare[1] = f[1 - cur];

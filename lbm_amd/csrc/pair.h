@@ -15,10 +15,12 @@ namespace {
 // Cells with the flag byte kSlowCell -- solid cells, column 0, the last row
 // (none of them streamed) and fluid cells whose eight upstream cells are all
 // solid (every population bounced) -- read only themselves over both steps.
-// The fixed-cell pass (fixed_cell over a list of them, launched first) steps
-// them into A2 and B2 and the strip kernel skips them: nothing pulls from them
-// but column 1 from column 0 and the row above from the last row, which the
-// strip kernel relays or recomputes.
+// The fixed-cell pass (fixed_cell over a list of them: the launch's blocks
+// after the strips, running alongside them) steps them into A2 and B2 and the
+// strip kernel skips them: nothing pulls from them but column 1 from column 0
+// and the row above from the last row, whose C_{n+1} the strip kernel
+// computes itself.  The two touch disjoint cells of A2 and B2 and only read
+// A and B.
 //
 // A block owns a strip of kBlock - 2 columns and `rows` rows; thread t works on
 // column x0 - 1 + t, so threads 0 and kBlock - 1 compute the strip's halo
@@ -29,15 +31,15 @@ namespace {
 // it (cells the stream does not write: the fixed-cell pass's); a call
 // therefore ends with one-step launches, which leave both sets whole.
 //
-// In the strip kernel the lane of column 0 only relays the three east-moving
-// populations of C_{n+1} from B2 into the ring, for column 1's pulls; with the
-// reference outlet the lane of column X relays the west-moving populations of
-// column 0 one row down, which the outlet's pulls reach through the flattened
-// index.  So no lane takes a per-cell path for the grid's edges or its solid
-// cells, and the only global loads not issued one row ahead are bounce-back
-// reads (a wave holding an edge or solid cell used to wait on dependent loads
-// in every row, and its block with it at every barrier).  Same per-cell
-// operations as k_cell, so the same bits.
+// In the strip kernel the lane of column 0 computes C_{n+1} of column 0 into
+// the ring, for column 1's pulls, and owns nothing; with the reference outlet
+// the lane of column X does the same with column 0 one row down, which the
+// outlet's pulls reach through the flattened index.  So no lane takes a
+// per-cell path for the grid's edges or its solid cells, and the only global
+// loads not issued one row ahead are bounce-back reads (a wave holding an edge
+// or solid cell used to wait on dependent loads in every row, and its block
+// with it at every barrier).  Same per-cell operations as k_cell, so the same
+// bits.
 constexpr int kRing = 4;
 
 // A kSlowCell cell (x, y) two steps on: C_{n+1} into B2 (where the cell is
@@ -179,77 +181,77 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
     const int t = (int)threadIdx.x;
     const int x = strip * (kBlock - 2) - 1 + t;
     const bool in_grid = x >= 0 && x < g.X;
-    // column 0 is the column-0 pass's: its lane only relays C_{n+1} (as does
-    // the lane of column X with the reference outlet)
+    // column 0 is the fixed-cell pass's: its lane only computes C_{n+1} for
+    // column 1's pulls (as does the lane of column X, one row down, for the
+    // reference outlet's); these relay lanes own nothing
     const bool owner = t >= 1 && t <= kBlock - 2 && x >= 1 && x < g.X;
     const bool lane0 = EDGE && x == 0, laneX = EDGE && wrap && x == g.X, outlet = EDGE && x == g.X - 1;
-    // Byte offsets of a lane's pulls from the pulled row of A, by the sign of
-    // c_x: the row's columns x + 1, x, x - 1, clamped into the row so every
-    // lane loads in bounds (lanes outside the grid, x = -1 or x >= nx, load and
-    // discard; the zero-gradient outlet's clamped upstream is its own column).
-    // The reference outlet's west-moving pulls wrap to column 0 of the next
-    // row (the flattened index): one row on, column 0; in the last rows that is
-    // the row past the set, which every set's allocation holds (those pulls are
-    // flagged as bounce-back).  The relay lanes read column 0 of B2, which the
-    // column-0 pass has written: column 0's lane the pulled row itself, column
-    // X's lane (reference outlet) the row after it.  The offsets are
-    // loop-invariant, so every lane issues the same loads.
+    const bool relay = lane0 || laneX;
+    // Byte offsets of a lane's nine loads from the pulled row of A (row r -
+    // c_y(k)).  Pulls by the sign of c_x: the row's columns x + 1, x, x - 1,
+    // clamped into the row so every lane loads in bounds (lanes outside the
+    // grid, x = -1 or x >= nx, load and discard; the zero-gradient outlet's
+    // clamped upstream is its own column).  The reference outlet's
+    // west-moving pulls wrap to column 0 of the next row (the flattened
+    // index): one row on, column 0; in the last rows that is up to two rows
+    // past the set, which every set's allocation holds (those pulls are
+    // flagged as bounce-back).  The relay lanes load S_{n+1} of column 0 --
+    // not streamed: B's moving populations and A's rest population -- of row r
+    // (column 0's lane) or r + 1 (column X's), so that phase 1 collides it
+    // like any other cell.  The offsets are loop-invariant, so every lane
+    // issues the same loads.
     const long long es = (long long)sizeof(T), row_b = g.ys * es;
-    const long long dB = (long long)((uintptr_t)B2 - (uintptr_t)A);
-    long long cw = (long long)min(max(x + 1, 0), g.X - 1) * es;
-    long long cc = (long long)min(max(x, 0), g.X - 1) * es;
-    long long ce = (long long)min(max(x - 1, 0), g.X - 1) * es;
-    if (lane0) ce = dB;
-    if (outlet && wrap) cw = row_b;
-    if (laneX) cw = dB + row_b;
-    // row 0's south-west pull from row -1 wraps to row 0 itself, the clamped
-    // row (only the first rows are loaded before the loop: `first`)
-    const long long cw_row0 = (outlet && wrap) || laneX ? cw - row_b : cw;
+    const long long dB = (long long)((uintptr_t)B - (uintptr_t)A);
+    long long off[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int ux = x - cx(k);
+        off[k] = (long long)min(max(ux, 0), g.X - 1) * es;
+        if (wrap && outlet && cx(k) < 0) off[k] = row_b;
+        if (lane0) off[k] = k == C ? 0 : dB + cy(k) * row_b;
+        if (laneX) off[k] = k == C ? row_b : dB + (1 + cy(k)) * row_b;
+    }
     const unsigned xc = (unsigned)min(max(x, 0), g.X - 1);
 
-    // phase 1, part 1: the nine pulls of row r, issued at clamped addresses
-    // (the rows are uniform: clamped once per block); patched in part 2
+    // phase 1, part 1: the nine loads of row r (the rows are uniform; the row
+    // below is clamped at row 0, the row above runs into the padding row)
     auto load1 = [&](int r, T v[9], uint32_t &fl, bool first) {
         if (r < 0 || r >= g.Y) return;
         fl = flags[(size_t)r * g.pitch + xc];
-        const char *up = (const char *)(A + (size_t)min(r + 1, g.Y - 1) * g.ys);  // cy = -1 pulls from r + 1
+        const char *up = (const char *)(A + (size_t)(r + 1) * g.ys);       // cy = -1 pulls from r + 1
         const char *mid = (const char *)(A + (size_t)r * g.ys);
-        const char *dn = (const char *)(A + (size_t)max(r - 1, 0) * g.ys);        // cy = +1 pulls from r - 1
+        const char *dn = (const char *)(A + (size_t)max(r - 1, 0) * g.ys);  // cy = +1 pulls from r - 1
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             const char *row = cy(k) < 0 ? up : (cy(k) > 0 ? dn : mid);
-            long long col = cx(k) < 0 ? cw : (cx(k) > 0 ? ce : cc);
-            if (first && cx(k) < 0 && cy(k) > 0 && r < 1) col = cw_row0;
-            v[k] = *(const T *)(row + k * ks * es + col);
+            long long o = off[k];
+            // row 0's loads from row -1 use the clamped row 0: the wrapped and
+            // relay loads there step one row back (only the first rows are
+            // loaded before the loop: `first`)
+            if (first && cy(k) > 0 && r < 1 && (relay || (wrap && outlet && cx(k) < 0))) o -= row_b;
+            v[k] = *(const T *)(row + k * ks * es + o);
         }
     };
     // phase 1, part 2: S_{n+1} of row r (bounce-back; the last row),
     // C_{n+1} = collide into the ring; returns the rest population.  A relay
-    // lane puts what it loaded, C_{n+1} of column 0 at the pulled row (column
-    // X's: the row after it), into the ring slot of the pulled row, where the
-    // neighbour's second-step pull finds it; the slots are those of the
-    // unclamped rows, and the clamped ones are never read.
+    // lane's loads are already the unstreamed S_{n+1} of column 0 (row r, or
+    // r + 1 for column X's lane, whose ring slot r the outlet's wrapped pulls
+    // read).
     auto finish1 = [&](int r, const T v[9], uint32_t fl) -> T {
         if (r < 0 || r >= g.Y) return T{};
-        if (lane0 || laneX) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k)
-                if ((cx(k) > 0 && lane0) || (cx(k) < 0 && laneX)) ring[(r - cy(k)) & (kRing - 1)][bit(k)][t] = v[k];
-            return T{};
-        }
         // fixed cells are the fixed-cell pass's, and nothing pulls from them
-        // but the row above from the last row
-        if (!in_grid || (fl == kSlowCell && r != g.Y - 1)) return T{};
+        // but the row above from the last row (and column 1 from column 0)
+        if (!relay && (!in_grid || (fl == kSlowCell && r != g.Y - 1))) return T{};
         const size_t sown = (size_t)r * g.ys + x;
         float f[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) f[k] = St::load(v[k], k, g.sh);
-        if (r == g.Y - 1) {
+        if (!relay && r == g.Y - 1) {
             // the last row is not streamed: the other set's moving populations
 #pragma unroll
             for (int k = 0; k < 9; ++k)
                 if (k != C) f[k] = St::load(B[(size_t)k * ks + sown], k, g.sh);
-        } else if (fl != 0) {
+        } else if (!relay && fl != 0) {
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 if (k == C) continue;
@@ -304,6 +306,15 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
     uint32_t fl = 0;
     load1(r0 - 1, v, fl, true);
     finish1(r0 - 1, v, fl);
+    if (r0 == 0 && laneX) {
+        // the outlet's south-west pull in row 0 wraps to column 0 of row 0:
+        // column X's slot of row -1
+        float f[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f[k] = St::load(k == C ? A[(size_t)C * ks] : B[(size_t)k * ks], k, g.sh);
+        St::collide(f, g.omega);
+        ring[(0 - 1) & (kRing - 1)][bit(SW)][t] = St::store(f[SW], SW, g.sh);
+    }
     load1(r0, v, fl, true);
     T rest = finish1(r0, v, fl);
     uint32_t fl_y = fl;
