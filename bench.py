@@ -54,7 +54,8 @@ def read_traffic(name):
 
 
 KERNEL_NAMES = {1: "k_cell (one step per launch)",
-                2: "k_pair + k_pair_col0 (two steps per launch: the state read and written once per pair of steps)"}
+                2: "k_pair (two steps per launch: the state read and written once per pair of steps; its last "
+                   "blocks step the cells that read only themselves)"}
 
 
 def updates_per_launch(variant):
@@ -95,8 +96,7 @@ def launch_us(g, variant, launches=40):
     """Average duration of one launch of the step kernel, from HIP events on the
     grid's stream.  A call of n steps with a two-step variant ends with one-step
     launches, so the two-step launch is timed as the difference of a call of
-    2 + 2m steps and one of 2 steps (m two-step launches, each with its column
-    kernel, which it needs)."""
+    2 + 2m steps and one of 2 steps (m two-step launches)."""
     if updates_per_launch(variant) == 2:
         g.step_timed(2)
         short = min(g.step_timed(2) for _ in range(3))
@@ -411,7 +411,8 @@ def main():
         a16 = 36.0 * cells / k16 / 1e9
         log(f"f16s: {m16:.0f} MLUPS, {k16 * 1e6:.1f} us per step, {u16} step(s) per launch, "
             f"{a16:.0f} GB/s algorithmic")
-        t16 = read_traffic("traffic_f16s_8192x4096.txt") if (X, Y) == (8192, 4096) and u16 == 1 else None
+        t16 = (read_traffic("traffic_f16s_8192x4096.txt" if u16 == 1 else "traffic_f16s_pair_8192x4096.txt")
+               if (X, Y) == (8192, 4096) else None)
         out["f16s"] = {
             "mlups": round(m16, 1),
             "speedup_vs_f32": round(m16 / mlups, 3),
