@@ -2,6 +2,8 @@
 // lbm_grid.hip) and the batched engine (k_batch_pair in lbm_batch.hip).
 #pragma once
 
+#include <type_traits>
+
 #include "lbm_common.h"
 
 namespace {
@@ -212,10 +214,14 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
         if (laneX) off[k] = k == C ? row_b : dB + (1 + cy(k)) * row_b;
     }
     const unsigned xc = (unsigned)min(max(x, 0), g.X - 1);
+    // a loaded value waits a row in a whole register: packing two binary16
+    // values into one made the compiler wait for the loads right after issuing
+    // them
+    using V = std::conditional_t<sizeof(T) == 2, uint32_t, T>;
 
     // phase 1, part 1: the nine loads of row r (the rows are uniform; the row
     // below is clamped at row 0, the row above runs into the padding row)
-    auto load1 = [&](int r, T v[9], uint32_t &fl, bool first) {
+    auto load1 = [&](int r, V v[9], uint32_t &fl, bool first) {
         if (r < 0 || r >= g.Y) return;
         fl = flags[(size_t)r * g.pitch + xc];
         const char *up = (const char *)(A + (size_t)(r + 1) * g.ys);       // cy = -1 pulls from r + 1
@@ -229,7 +235,7 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
             // relay loads there step one row back (only the first rows are
             // loaded before the loop: `first`)
             if (first && cy(k) > 0 && r < 1 && (relay || (wrap && outlet && cx(k) < 0))) o -= row_b;
-            v[k] = *(const T *)(row + k * ks * es + o);
+            v[k] = (V) * (const T *)(row + k * ks * es + o);
         }
     };
     // phase 1, part 2: S_{n+1} of row r (bounce-back; the last row),
@@ -237,7 +243,7 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
     // lane's loads are already the unstreamed S_{n+1} of column 0 (row r, or
     // r + 1 for column X's lane, whose ring slot r the outlet's wrapped pulls
     // read).
-    auto finish1 = [&](int r, const T v[9], uint32_t fl) -> T {
+    auto finish1 = [&](int r, const V v[9], uint32_t fl) -> T {
         if (r < 0 || r >= g.Y) return T{};
         // fixed cells are the fixed-cell pass's, and nothing pulls from them
         // but the row above from the last row (and column 1 from column 0)
@@ -245,7 +251,7 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
         const size_t sown = (size_t)r * g.ys + x;
         float f[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) f[k] = St::load(v[k], k, g.sh);
+        for (int k = 0; k < 9; ++k) f[k] = St::load((T)v[k], k, g.sh);
         if (!relay && r == g.Y - 1) {
             // the last row is not streamed: the other set's moving populations
 #pragma unroll
@@ -302,7 +308,7 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
 
     // rows r0 - 1 and r0, then per row y: row y + 2's pulls issued, row y + 1
     // finished into the ring, barrier, row y's second step
-    T v[9];
+    V v[9];
     uint32_t fl = 0;
     load1(r0 - 1, v, fl, true);
     finish1(r0 - 1, v, fl);
@@ -320,7 +326,7 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
     uint32_t fl_y = fl;
     load1(r0 + 1, v, fl, true);
     for (int y = r0; y < r1; ++y) {
-        T cv[9];
+        V cv[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) cv[k] = v[k];
         const uint32_t cfl = fl;
