@@ -160,6 +160,27 @@ __device__ __forceinline__ void fixed_row_emit(const uint8_t *__restrict__ row, 
     }
 }
 
+// Raw buffer loads (gfx9 buffer descriptor, word 3 0x00020000: 32-bit
+// untyped data; offsets = lane offset + uniform offset, bounds-checked).
+constexpr int kBufferWord3 = 0x00020000;
+
+template <class T>
+__device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    if constexpr (sizeof(T) == 2)
+        return (T)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+// non-temporal (cache policy bit 1) raw buffer store
+template <class T>
+__device__ __forceinline__ void buffer_store_nt(T v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    if constexpr (sizeof(T) == 2)
+        __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, voff, soff, 2);
+    else
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 2);
+}
+
 // Barrier for LDS hand-off only: waits for this wave's LDS operations, not for
 // its global loads, so the next row's pulls stay in flight across it.
 __device__ __forceinline__ void lds_barrier() {
@@ -219,6 +240,15 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
     // them
     using V = std::conditional_t<sizeof(T) == 2, uint32_t, T>;
 
+    // the strip's rows of A and A2 from row brow as buffers (blocks away from
+    // the edges load and store through them): the first row loaded is r0 - 1,
+    // whose southward pulls come from r0 - 2
+    const int brow = max(r0 - 2, 0);
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(A + (size_t)brow * g.ys), (short)0, 0x7FFFFFFF, kBufferWord3);
+    const __amdgpu_buffer_rsrc_t rsrc2 =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(A2 + (size_t)brow * g.ys), (short)0, 0x7FFFFFFF, kBufferWord3);
+
     // phase 1, part 1: the nine loads of row r (the rows are uniform; the row
     // below is clamped at row 0, the row above runs into the padding row)
     auto load1 = [&](int r, V v[9], uint32_t &fl, bool first) {
@@ -235,7 +265,17 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
             // relay loads there step one row back (only the first rows are
             // loaded before the loop: `first`)
             if (first && cy(k) > 0 && r < 1 && (relay || (wrap && outlet && cx(k) < 0))) o -= row_b;
-            v[k] = (V) * (const T *)(row + k * ks * es + o);
+            if constexpr (EDGE) {
+                v[k] = (V) * (const T *)(row + k * ks * es + o);
+            } else {
+                // away from the edges every pull is a column of a row near the
+                // block: a buffer load with the row and plane in the uniform
+                // offset and the column in the lane offset (no 64-bit lane
+                // address per load)
+                const int pr = cy(k) < 0 ? r + 1 : (cy(k) > 0 ? max(r - 1, 0) : r);
+                const uint32_t soff = (uint32_t)((long long)(pr - brow) * row_b + k * ks * es);
+                v[k] = (V)buffer_load<T>(rsrc, (uint32_t)o, soff);
+            }
         }
     };
     // phase 1, part 2: S_{n+1} of row r (bounce-back; the last row),
@@ -261,7 +301,15 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 if (k == C) continue;
-                if ((fl >> bit(k)) & 1u) f[k] = St::load(A[(size_t)opp(k) * ks + sown], opp(k), g.sh);
+                if ((fl >> bit(k)) & 1u) {
+                    T b;
+                    if constexpr (EDGE)
+                        b = A[(size_t)opp(k) * ks + sown];
+                    else
+                        b = buffer_load<T>(rsrc, (uint32_t)x * (uint32_t)es,
+                                           (uint32_t)((long long)(r - brow) * row_b + opp(k) * ks * es));
+                    f[k] = St::load(b, opp(k), g.sh);
+                }
             }
         }
         St::collide(f, g.omega);
@@ -301,9 +349,16 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
             }
         }
         St::collide(f, g.omega);
-        T *dst = A2 + (size_t)y * g.ys;
+        if constexpr (EDGE) {
+            T *dst = A2 + (size_t)y * g.ys;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(St::store(f[k], k, g.sh), dst + (size_t)k * ks + x);
+            for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(St::store(f[k], k, g.sh), dst + (size_t)k * ks + x);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                buffer_store_nt<T>(St::store(f[k], k, g.sh), rsrc2, (uint32_t)x * (uint32_t)es,
+                                   (uint32_t)((long long)(y - brow) * row_b + k * ks * es));
+        }
     };
 
     // rows r0 - 1 and r0, then per row y: row y + 2's pulls issued, row y + 1
