@@ -37,11 +37,11 @@ namespace {
 // the ring, for column 1's pulls, and owns nothing; with the reference outlet
 // the lane of column X does the same with column 0 one row down, which the
 // outlet's pulls reach through the flattened index.  So no lane takes a
-// per-cell path for the grid's edges or its solid cells, and the only global
-// loads not issued one row ahead are bounce-back reads (a wave holding an edge
-// or solid cell used to wait on dependent loads in every row, and its block
-// with it at every barrier).  Same per-cell operations as k_cell, so the same
-// bits.
+// per-cell path for the grid's edges or its solid cells, and the only loads not
+// issued one row ahead are bounce-back reads and the last row's (a wave holding
+// an edge or solid cell used to wait on dependent loads in every row, and its
+// block with it at every barrier).  Same per-cell operations as k_cell, so the
+// same bits.
 constexpr int kRing = 4;
 
 // A kSlowCell cell (x, y) two steps on: C_{n+1} into B2 (where the cell is
