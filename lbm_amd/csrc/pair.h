@@ -92,10 +92,11 @@ __device__ __forceinline__ void fixed_cell(const typename St::T *A, const typena
 //
 // The fixed-cell pass runs one thread per entry of a list of the kSlowCell
 // cells, (column and whether it is streamed, row) in row order, built on the
-// device after each mask change: a count per row, an exclusive scan of the counts on the host, and
-// an ordered emit per row.  One block of kFixedBlock threads per row; each
-// thread looks at 16 flag bytes (one aligned 16-byte word: the flag plane's
-// rows are padded to a multiple of 64 bytes) per pass over the row.
+// device after each mask change: a count per row, an exclusive scan of the
+// counts on the host, and an ordered emit per row.  One block of kFixedBlock
+// threads per row; each thread looks at 16 flag bytes (one aligned 16-byte
+// word: the flag plane's rows are padded to a multiple of 64 bytes) per pass
+// over the row.
 constexpr int kFixedBlock = 256;
 
 // bit i set: column x0 + i < X is a kSlowCell cell
