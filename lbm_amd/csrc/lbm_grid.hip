@@ -290,6 +290,7 @@ struct lbm_grid {
     uint2 *fixed = nullptr;               // its fixed-cell list: (column | streamed << 31, row) per kSlowCell cell
     uint32_t *fixed_rows = nullptr;       // the list's offset per row
     uint32_t n_fixed = 0;
+    bool fixed_dirty = true;              // the list does not follow the flags (a rebuild failed)
     int cur = 0;
     bool pform = false;
     uint8_t *mask = nullptr, *flags = nullptr;
@@ -439,8 +440,10 @@ struct lbm_grid {
     }
 
     // the fixed-cell list from the current flags (after every mask change
-    // while the two-step buffers exist); waits for the stream
+    // while the two-step buffers exist); waits for the stream.  Until it has
+    // succeeded the list is dirty, and a two-step launch rebuilds it first.
     void build_fixed_list() {
+        fixed_dirty = true;
         hipLaunchKernelGGL(k_fixed_count, dim3(Y), dim3(kFixedBlock), 0, stream, flags, geom, fixed_rows);
         HIPCHK(hipGetLastError());
         std::vector<uint32_t> off(Y);
@@ -457,12 +460,14 @@ struct lbm_grid {
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream));  // `off` is read by the copy above
         n_fixed = total;
+        fixed_dirty = false;
     }
 
     template <class St, int MODE>
     void launch_pair() {
         using T = typename St::T;
         ensure_pair_buffers();
+        if (fixed_dirty) build_fixed_list();
         const T *A = (const T *)f[cur], *B = (const T *)f[1 - cur];
         T *A2 = (T *)spare[0], *B2 = (T *)spare[1];
         const int rows = pair_rows();
