@@ -188,6 +188,11 @@ __device__ __forceinline__ void lds_barrier() {
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// Waits for all of this wave's global loads (s_waitcnt vmcnt(0); expcnt and
+// lgkmcnt left alone).  A builtin, not asm: the compiler's own wait insertion
+// sees it, so a branch that ends with it leaves no load pending at its join.
+__device__ __forceinline__ void wait_global_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // One block's work of the two-step kernel: the strip of kBlock - 2 owner
 // columns from strip * (kBlock - 2), rows [r0, r1), with `ring` (C_{n+1} of
 // four rows: the eight moving populations; the rest population is only read
@@ -299,19 +304,25 @@ __device__ __forceinline__ void pair_block(typename St::T (&ring)[kRing][8][kBlo
             for (int k = 0; k < 9; ++k)
                 if (k != C) f[k] = St::load(B[(size_t)k * ks + sown], k, g.sh);
         } else if (!relay && fl != 0) {
+            // the cell's own eight moving populations, all loaded, then the
+            // bounced ones selected: loads under per-direction branches were
+            // waited for one by one, and one left pending past the branch made
+            // every wave wait at the join for all its loads in flight, the
+            // next row's pulls included
+            T b[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 if (k == C) continue;
-                if ((fl >> bit(k)) & 1u) {
-                    T b;
-                    if constexpr (EDGE)
-                        b = A[(size_t)opp(k) * ks + sown];
-                    else
-                        b = buffer_load<T>(rsrc, (uint32_t)x * (uint32_t)es,
-                                           (uint32_t)((long long)(r - brow) * row_b + opp(k) * ks * es));
-                    f[k] = St::load(b, opp(k), g.sh);
-                }
+                if constexpr (EDGE)
+                    b[k] = A[(size_t)opp(k) * ks + sown];
+                else
+                    b[k] = buffer_load<T>(rsrc, (uint32_t)x * (uint32_t)es,
+                                          (uint32_t)((long long)(r - brow) * row_b + opp(k) * ks * es));
             }
+            wait_global_loads();
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                if (k != C && ((fl >> bit(k)) & 1u)) f[k] = St::load(b[k], opp(k), g.sh);
         }
         St::collide(f, g.omega);
         T o[9];
