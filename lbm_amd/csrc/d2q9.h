@@ -117,14 +117,12 @@ inline void host_feq(float ux, float uy, float rho, float out[9]) {
 // full f32 resolution.  Used with the f16 storage, whose error budget is stated
 // against an FP64 solver; it is not the reference's expression tree.
 __host__ __device__ __forceinline__ Moments moments_shifted(const float g[9], bool with_rest = true) {
-    // the east and west columns of the stencil give j_x and, with N and S, the
-    // density; the north and south rows give j_y
-    const float east = (g[NE] + g[SE]) + g[E], west = (g[NW] + g[SW]) + g[W];
-    const float north = (g[NW] + g[NE]) + g[N], south = (g[SW] + g[SE]) + g[S];
-    float drho = (east + west) + (g[N] + g[S]);
+    const float corner = ((g[NW] + g[NE]) + (g[SW] + g[SE]));
+    const float axis = ((g[N] + g[S]) + (g[W] + g[E]));
+    float drho = corner + axis;
     if (with_rest) drho += g[C];
-    const float jx = east - west;
-    const float jy = north - south;
+    const float jx = ((g[NE] + g[E]) + g[SE]) - ((g[NW] + g[W]) + g[SW]);
+    const float jy = ((g[NW] + g[N]) + g[NE]) - ((g[SW] + g[S]) + g[SE]);
     // rho without the rest population: 1 - w_C + the shifted moving populations
     const float rho = with_rest ? 1.0f + drho : (float)(5.0 / 9.0) + drho;
     return {jx, jy, rho};
@@ -137,23 +135,23 @@ __device__ __forceinline__ Moments collide_shifted(float g[9], float omega) {
     const float ux = m.jx * inv;
     const float uym = -m.jy * inv;  // memory-row velocity: north is row - 1
     const float base = fmaf(-1.5f * m.rho, fmaf(ux, ux, uym * uym), drho);
-    // g_k <- (1 - omega) g_k + omega geq_k, with omega folded into the
-    // coefficients: omega geq_k = cu (4.5 w_k omega rho cu + 3 w_k omega rho)
-    // + w_k omega base (per weight class two products and one per population,
-    // and three operations per moving population)
-    const float orho = omega * m.rho, obase = omega * base, keep = 1.0f - omega;
+    const float r3 = 3.0f * m.rho, r45 = 4.5f * m.rho;
+    // g + omega (geq - g), not (1 - omega) g + omega geq: near the free stream
+    // geq - g is small and exact, and only its product is rounded, where the
+    // other form rounds at the scale of g (the F16E format, whose stored
+    // deviations there are tiny, lost accuracy against FP64 with it)
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         // c_k . u with its zero terms left out (the compiler keeps 0 * u, whose
         // sign follows u's): the rest population's geq is w_C * base
-        float oeq = weight(k) * obase;
+        float geq = weight(k) * base;
         if (cx(k) != 0 || cy(k) != 0) {
             const float cu = cx(k) == 0   ? (float)cy(k) * uym
                              : cy(k) == 0 ? (float)cx(k) * ux
                                           : (float)cx(k) * ux + (float)cy(k) * uym;
-            oeq = fmaf(cu, fmaf((4.5f * weight(k)) * orho, cu, (3.0f * weight(k)) * orho), oeq);
+            geq = weight(k) * fmaf(cu, fmaf(r45, cu, r3), base);
         }
-        g[k] = fmaf(keep, g[k], oeq);
+        g[k] = fmaf(omega, geq - g[k], g[k]);
     }
     return m;
 }
