@@ -481,8 +481,9 @@ def test_two_step_variants_whole_state_and_paint_between_calls(variant, precisio
     b.close()
 
 
-@pytest.mark.parametrize("X,Y", [(3, 3), (5, 4), (64, 9), (254, 17), (255, 8), (256, 3), (509, 12), (512, 25),
-                                 (507, 10), (508, 9), (510, 19), (1017, 7), (1018, 33)])
+@pytest.mark.parametrize("X,Y", [(3, 3), (5, 4), (64, 9), (254, 17), (255, 8), (256, 3), (257, 6), (258, 5),
+                                 (509, 12), (512, 25), (513, 4), (507, 10), (508, 9), (510, 19), (1017, 7),
+                                 (1018, 33), (1024, 6), (1025, 9)])
 def test_two_step_variants_small_and_boundary_sizes(X, Y):
     """Grids at and around the strip width (254 columns), the row block (8 or 16
     rows) and the row pitch (a multiple of 64, where the east pull is clamped at
@@ -513,7 +514,8 @@ def test_two_step_variants_small_and_boundary_sizes(X, Y):
 
 
 @pytest.mark.parametrize("X,Y,fill", [(254, 19, 0.25), (255, 17, 0.25), (509, 33, 0.3), (64, 9, 0.2),
-                                      (1018, 21, 0.25), (300, 12, 0.0)])
+                                      (1018, 21, 0.25), (300, 12, 0.0), (256, 13, 0.25), (257, 11, 0.25),
+                                      (513, 9, 0.3)])
 def test_two_step_variants_open_rows_and_random_obstacles(X, Y, fill):
     """No wall rows: the first and last rows are fluid, so the outlet's pulls wrap
     between rows at the grid's top and bottom (row 0's south-west pull wraps to
